@@ -1,0 +1,214 @@
+"""Pins the CPU oracle (oracle/v2rdm_oracle.c) to the reference's algorithm.
+
+Each check is chosen so that a plausible restatement mistake (a dropped term,
+a wrong sign or index, a transposed operand, a wrong block size) fails it:
+  * layout sizes against the counts the reference's own outputs print;
+  * A.x = b for exact RDMs of brute-force FCI states, every block built as a
+    Gram matrix of Fock-space vectors (never from the constraint formulas);
+  * c.x = E_FCI (cost vector, tei.cc:144-313);
+  * adjointness of A^T against A;
+  * CG against a dense least-squares solve;
+  * Update_xz against its closed form;
+  * the full loop: N=2 and 2-hole DQG energies equal FCI (N-representability is
+    exact there), any N gives a lower bound.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import fci_tools as ft
+import oracle_py as orc
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+LAYOUTS = [
+    # amopi, nalpha, nbeta, multiplicity, positivity, constrain_sz
+    ([4], 2, 2, 1, "DQG", True),
+    ([4], 2, 1, 2, "DQG", True),
+    ([5], 3, 1, 3, "DQG", True),
+    ([2, 1, 1, 0], 2, 2, 1, "DQG", True),
+    ([1, 1, 1, 1, 0, 1, 0, 0], 2, 2, 1, "DQG", True),
+    ([4], 2, 2, 1, "DQG", False),
+    ([4], 2, 2, 1, "DQ", True),
+    ([4], 2, 2, 1, "DG", True),
+    ([4], 2, 2, 1, "DQGT2", True),
+    ([5], 3, 2, 2, "DQGT2", True),
+    ([2, 1, 1, 1], 2, 2, 1, "DQGT2", True),
+]
+
+
+def _layout(amopi, na, nb, mult, pos, sz=True):
+    return orc.Layout(amopi, na, nb, mult, pos, constrain_sz=sz)
+
+
+def _ham(L, seed=0):
+    n = L.amo
+    irreps = [L.irrep(p) for p in range(n)]
+    return ft.random_hamiltonian(n, seed, irreps if L.nirrep > 1 else None)
+
+
+def test_layout_sizes_match_reference_outputs():
+    cases = json.load(open(os.path.join(HERE, "golden", "layout_sizes.json")))
+    for cs in cases:
+        L = orc.Layout(cs["amopi"], cs["nalpha"], cs["nbeta"], cs["multiplicity"],
+                       cs["positivity"])
+        assert L.dimx == cs["dimx"], cs["source"]
+        extra = 0
+        if cs["year"] < 2018 and "G" in cs["positivity"]:
+            extra = 2 * L.gems_ab(0)   # maximal-spin rows added after 2017
+        assert L.ncon == cs["ncon"] + extra, cs["source"]
+
+
+@pytest.mark.parametrize("amopi,na,nb,mult,pos,sz", LAYOUTS)
+def test_exact_rdms_are_feasible(amopi, na, nb, mult, pos, sz):
+    L = _layout(amopi, na, nb, mult, pos, sz)
+    h, eri = _ham(L)
+    F = ft.Fock(L.amo)
+    E, psi = F.ground_state(h, eri, na, nb, s_target=(na - nb) / 2)
+    x = ft.exact_x(F, psi, L)
+    b = L.build_b()
+    r = L.Au(x) - b
+    for name, st, cnt in L.row_families():
+        assert np.abs(r[st:st + cnt]).max() < 1e-11, name
+    c = L.build_c(h, eri)
+    assert abs(c @ x - E) < 1e-10
+
+
+def test_feasibility_detects_a_planted_sign_error():
+    """The feasibility check must fail when one family's coefficient is wrong:
+    flip the sign of the D2aa block of x (equivalent to a sign slip in every
+    row that reads it) and expect residuals."""
+    L = _layout([4], 2, 2, 1, "DQG")
+    h, eri = _ham(L)
+    F = ft.Fock(L.amo)
+    E, psi = F.ground_state(h, eri, 2, 2, s_target=0)
+    x = ft.exact_x(F, psi, L)
+    o = L.offset("D2AA", 0)
+    g = L.gems_aa(0)
+    x[o:o + g * g] *= -1.0
+    assert np.abs(L.Au(x) - L.build_b()).max() > 1e-3
+
+
+@pytest.mark.parametrize("amopi,na,nb,mult,pos,sz", LAYOUTS)
+def test_adjointness(amopi, na, nb, mult, pos, sz):
+    L = _layout(amopi, na, nb, mult, pos, sz)
+    rng = np.random.default_rng(1)
+    x = rng.normal(size=L.dimx)
+    y = rng.normal(size=L.ncon)
+    lhs = L.Au(x) @ y
+    rhs = x @ L.ATu(y)
+    assert abs(lhs - rhs) < 1e-12 * (abs(lhs) + np.linalg.norm(x) * np.linalg.norm(y))
+
+
+def test_cg_matches_dense_solve():
+    L = _layout([3], 1, 1, 1, "DQG")
+    rng = np.random.default_rng(2)
+    A = np.stack([L.Au(e) for e in np.eye(L.dimx)], axis=1)   # ncon x dimx
+    rhs = A @ rng.normal(size=L.dimx)                          # consistent rhs
+    y, it = L.cg_solve(rhs, np.zeros(L.ncon), 1e-11, 5000)
+    AAT = A @ A.T
+    assert np.linalg.norm(AAT @ y - rhs) < 1e-9
+    # A^T y is unique even though A A^T is singular (dependent rows)
+    ylsq = np.linalg.lstsq(AAT, rhs, rcond=None)[0]
+    assert np.allclose(A.T @ y, A.T @ ylsq, atol=1e-8)
+
+
+def test_update_xz_closed_form():
+    L = _layout([2, 1, 1, 0], 2, 2, 1, "DQG")
+    rng = np.random.default_rng(3)
+    y = rng.normal(size=L.ncon)
+    c = rng.normal(size=L.dimx)
+    x0 = rng.normal(size=L.dimx)
+    mu = 0.7
+    x, z = L.update_xz(mu, y, c, x0)
+    M = L.ATu(y) - c + mu * x0
+    off = 0
+    for n in L.block_dims:
+        if n == 0:
+            continue
+        blk = M[off:off + n * n].reshape(n, n)
+        S = 0.5 * (blk + blk.T)
+        xb = x[off:off + n * n].reshape(n, n)
+        zb = z[off:off + n * n].reshape(n, n)
+        w, V = np.linalg.eigh(S)
+        assert np.allclose(mu * xb - zb, S, atol=1e-12)
+        assert np.allclose(xb, (V * np.maximum(w, 0)) @ V.T / mu, atol=1e-12)
+        assert np.linalg.eigvalsh(0.5 * (xb + xb.T)).min() > -1e-12
+        assert np.linalg.eigvalsh(0.5 * (zb + zb.T)).min() > -1e-12
+        assert np.abs(xb @ zb).max() < 1e-10
+        off += n * n
+
+
+def test_random_guess_is_glibc_rand_sequence():
+    """srand(0); rand() -- glibc treats seed 0 as 1; first outputs are the
+    published glibc TYPE_3 values 1804289383, 846930886, 1681692777."""
+    L = _layout([2], 1, 1, 1, "D")
+    x, z, y = L.guess_random()
+    RAND_MAX = 2147483647
+    first = [1804289383, 846930886, 1681692777]
+    assert np.isclose(x[0], (first[0] / RAND_MAX - 1.0) * 2.0)
+    assert np.isclose(z[0], (first[1] / RAND_MAX - 1.0) * 2.0)
+    assert np.isclose(x[1], (first[2] / RAND_MAX - 1.0) * 2.0)
+    assert x.min() >= -2.0 and x.max() <= 0.0
+
+
+def _loop_energy(amopi, na, nb, pos, seed=0, conv=1e-5):
+    L = orc.Layout(amopi, na, nb, None, pos)
+    h, eri = _ham(L, seed)
+    F = ft.Fock(L.amo)
+    E, _ = F.ground_state(h, eri, na, nb, s_target=(na - nb) / 2)
+    c = L.build_c(h, eri)
+    b = L.build_b()
+    x, z, y = L.guess_random()
+    st = L.bpsdp(c, b, x, z, y, r_conv=conv, e_conv=conv, maxiter=20000)
+    return E, st
+
+
+def test_two_electrons_dqg_is_exact():
+    E, st = _loop_energy([4], 1, 1, "DQG")
+    assert st["converged"]
+    assert abs(st["energy_primal"] - E) < 1e-4
+
+
+def test_two_electrons_d_is_exact():
+    E, st = _loop_energy([4], 1, 1, "D")
+    assert st["converged"]
+    assert abs(st["energy_primal"] - E) < 1e-4
+
+
+def test_two_holes_dqg_is_exact():
+    E, st = _loop_energy([4], 3, 3, "DQG")
+    assert st["converged"]
+    assert abs(st["energy_primal"] - E) < 1e-4
+
+
+def test_mlp_gradient_matches_finite_differences():
+    rng = np.random.default_rng(4)
+    n, nin, nh = 13, 6, 8
+    f = rng.normal(size=(n, nin)).astype(np.float32)
+    t = rng.normal(size=n).astype(np.float32)
+    wt = rng.uniform(0.5, 1.5, n).astype(np.float32)
+    W1 = rng.normal(0, 0.5, (nin, nh)).astype(np.float32)
+    b1 = rng.normal(0, 0.1, nh).astype(np.float32)
+    W2 = rng.normal(0, 0.5, (nh, nh)).astype(np.float32)
+    b2 = rng.normal(0, 0.1, nh).astype(np.float32)
+    w3 = rng.normal(0, 0.5, nh).astype(np.float32)
+    b3 = np.float32(0.1)
+    loss, g = orc.mlp_loss_grad(f, t, wt, W1, b1, W2, b2, w3, b3)
+    out = orc.mlp_forward(f, W1, b1, W2, b2, w3, b3)
+    assert abs(loss - np.mean(wt * (out - t) ** 2)) < 1e-5
+    params = [W1, b1, W2, b2, w3]
+    for pi, P in enumerate(params):
+        flat = P.reshape(-1)
+        for k in range(0, flat.size, max(1, flat.size // 5)):
+            eps = 1e-2
+            old = flat[k]
+            flat[k] = old + eps
+            lp = orc.mlp_loss_grad(f, t, wt, W1, b1, W2, b2, w3, b3)[0]
+            flat[k] = old - eps
+            lm = orc.mlp_loss_grad(f, t, wt, W1, b1, W2, b2, w3, b3)[0]
+            flat[k] = old
+            fd = (lp - lm) / (2 * eps)
+            assert abs(fd - g[pi][k]) < 2e-3 * (1 + abs(fd)), (pi, k)
