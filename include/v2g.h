@@ -1,0 +1,127 @@
+/*
+ * v2g.h -- C ABI of the MI355X boundary-point SDP (BPSDP) engine of
+ * v2RDM-CASSCF.  Drop-in for the hot path of v2RDMSolver (v2rdm_solver.h):
+ * the constraint maps, the CG solve of A A^T y = rhs, the per-block eigen
+ * split and the outer iteration.
+ *
+ * Conventions: plain C, int status (0 = success, negative = error, message
+ * from v2g_last_error()), no exceptions cross the boundary.  Vector arguments
+ * named *_dev are device pointers (hipMalloc / torch CUDA tensors) holding
+ * FP64 data in the reference's layout (x: dimx entries in dimensions_ block
+ * order; y: ncon entries in bpsdp_Au row order).  Calls run on the handle's
+ * stream and return after it has drained, unless stated otherwise.
+ */
+#ifndef V2G_H
+#define V2G_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct v2g_solver *v2g_handle;
+
+/* Problem description; mirrors what v2RDMSolver::common_init derives from
+ * Psi4 options (v2rdm_solver.cc:488-545 POSITIVITY -> constrain_*,
+ * CONSTRAIN_SPIN, CONSTRAIN_SZ; plugin.cc:102-114). */
+typedef struct v2g_layout_desc {
+    int nirrep;            /* 1, 2, 4 or 8 (D2h subgroup, Cotton order)      */
+    int amopi[8];          /* active orbitals per irrep                      */
+    int nalpha, nbeta;     /* active electrons                               */
+    int multiplicity;      /* 2S+1                                           */
+    int constrain_q2, constrain_g2, constrain_t2;
+    int constrain_spin, constrain_sz;
+} v2g_layout_desc;
+
+/* Replaces v2RDMSolver construction + common_init sizes (v2rdm_solver.cc:
+ * 129-1675): builds the geminal tables (basis.cc:439-757), block offsets and
+ * the constraint-row plan on the device. */
+int v2g_create(const v2g_layout_desc *desc, v2g_handle *out);
+int v2g_destroy(v2g_handle h);
+/* dimx_, nconstraints_, dimensions_.size() (v2rdm_solver.h:161-340) */
+int v2g_dims(v2g_handle h, long long *dimx, long long *ncon, int *nblocks);
+/* dimensions_ (v2rdm_solver.cc:1129-1270) */
+int v2g_block_dims(v2g_handle h, int *dims);
+/* run on this HIP stream instead of the handle's own (NULL = own) */
+int v2g_set_stream(v2g_handle h, void *hip_stream);
+
+/* v2RDMSolver::bpsdp_Au(A, u) (v2rdm_solver.cc:3020-3065): Ax = A x */
+int v2g_Au(v2g_handle h, const double *x_dev, double *Ax_dev);
+/* v2RDMSolver::bpsdp_ATu(A, u) (v2rdm_solver.cc:3114-3158): ATy = A^T y */
+int v2g_ATu(v2g_handle h, const double *y_dev, double *ATy_dev);
+/* v2RDMSolver::cg_Ax (v2rdm_solver.cc:3207-3213): out = A (A^T u) */
+int v2g_cg_Ax(v2g_handle h, const double *u_dev, double *out_dev);
+/* diag(A A^T): squared norm of every constraint row (no reference
+ * counterpart; the Jacobi preconditioner) */
+int v2g_row_norm2(v2g_handle h, double *d_dev);
+
+/* CGSolver::solve (cg_solver.cc:125-177) with evaluate_Ap = cg_Ax:
+ * warm start from y, stop when ||r||_2 < tol or after maxiter iterations.
+ * precond: 0 none (the reference), 1 Jacobi 1/diag(A A^T), 2 the diagonal
+ * given to v2g_set_precond_diag (learned); flexible != 0 uses the
+ * Polak-Ribiere beta.  iters receives CGSolver::total_iterations(). */
+int v2g_cg_solve(v2g_handle h, const double *rhs_dev, double *y_dev, double tol, int maxiter,
+                 int precond, int flexible, int *iters, double *resnorm);
+/* inverse-diagonal used by precond = 2 (ncon entries, device) */
+int v2g_set_precond_diag(v2g_handle h, const double *minv_dev);
+
+/* v2RDMSolver::Update_xz (v2rdm_solver.cc:3216-3289): M = A^T y - c + mu x,
+ * per block x = P+(M)/mu, z = -P-(M); x is read and overwritten.  The
+ * eigenvectors are kept for the next call (warm start). */
+int v2g_update_xz(v2g_handle h, double mu, const double *y_dev, const double *c_dev,
+                  double *x_dev, double *z_dev);
+/* forget the kept eigenvectors (next update_xz starts cold) */
+int v2g_reset_eig(v2g_handle h);
+
+typedef struct v2g_params {
+    double tau;                  /* TAU_PARAMETER (plugin.cc:97)          */
+    double r_convergence;        /* R_CONVERGENCE                         */
+    double e_convergence;        /* E_CONVERGENCE                         */
+    double cg_convergence;       /* CG_CONVERGENCE                        */
+    int maxiter;                 /* MAXITER                               */
+    int cg_maxiter;              /* CG_MAXITER                            */
+    int mu_update_frequency;     /* MU_UPDATE_FREQUENCY                   */
+    int precond;                 /* as v2g_cg_solve                       */
+    int flexible;
+    int verbose;                 /* 1: one line per outer iteration       */
+    double max_seconds;          /* > 0: stop after this wall time        */
+} v2g_params;
+
+typedef struct v2g_stats {
+    int oiter;                   /* outer iterations done                 */
+    long long cg_iters;          /* total CG iterations                   */
+    int converged;
+    double energy_primal, energy_dual, ep, ed, mu;
+    double t_cg, t_xz, t_total;  /* seconds: CG phase, Update_xz, loop    */
+    int eig_sweeps;              /* block-Jacobi sweeps, last Update_xz   */
+} v2g_stats;
+
+/* v2RDMSolver::compute_energy outer loop (v2rdm_solver.cc:1778-1912, orbital
+ * optimization off): x, z, y in/out (the guess on entry), c and b inputs,
+ * all device-resident for the whole solve.  Energies exclude enuc/efzc. */
+int v2g_bpsdp(v2g_handle h, const v2g_params *p, const double *c_dev, const double *b_dev,
+              double *x_dev, double *z_dev, double *y_dev, v2g_stats *st);
+
+/* callback invoked after each outer iteration inside v2g_bpsdp when set
+ * (hook for the learned preconditioner's online training); return nonzero
+ * to stop the loop */
+typedef int (*v2g_iter_callback)(void *user, int oiter, const v2g_stats *st);
+int v2g_set_iter_callback(v2g_handle h, v2g_iter_callback cb, void *user);
+
+/* Path inputs, built on the host (one-time work):
+ * v2RDMSolver::BuildConstraints (v2rdm_solver.cc:2518-3017): b, ncon entries */
+int v2g_build_b(v2g_handle h, double *b_host);
+/* RepackIntegrals + FrozenCoreEnergy without core orbitals (tei.cc:144-313):
+ * h1 = amo x amo one-electron integrals, eri = amo^4 (pq|rs) (chemists'
+ * notation), both in Pitzer order, row-major; c has dimx entries */
+int v2g_build_c(v2g_handle h, const double *h1_host, const double *eri_host, double *c_host);
+/* TPDM_GUESS RANDOM (v2rdm_solver.cc:2479-2490): srand(0), then rand() for
+ * x, z (interleaved) and y, each in [-2, 0] */
+int v2g_guess_random(v2g_handle h, double *x_host, double *z_host, double *y_host);
+
+const char *v2g_last_error(void);
+const char *v2g_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,218 @@
+// emu.cc -- bounds-checked host execution of the device row code
+// (v2rdm_casscf_amd/csrc/rowterms.hpp) for the CPU test suite.
+//
+// Test infrastructure: it runs the exact functions the HIP kernels run, so
+// every primal / constraint index they form is checked on the CPU against
+// the buffers, and their results against the oracle, before any GPU launch.
+// x reads go through a checking functor; y reads land in NaN-poisoned guard
+// regions when out of range, which poisons the result.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <vector>
+
+#include "../../v2rdm_casscf_amd/csrc/devlayout.hpp"
+#include "../../v2rdm_casscf_amd/csrc/layout.hpp"
+#include "../../v2rdm_casscf_amd/csrc/rowterms.hpp"
+
+using namespace v2g;
+
+struct Emu {
+    Layout L;
+    DevLayout D{};
+    long long violations = 0;
+    explicit Emu(const Options &o) : L(o)
+    {
+        D.nirrep = L.nirrep;
+        D.amo = L.amo;
+        D.singlet = L.singlet;
+        D.sz = o.sz;
+        D.spin = o.spin;
+        D.q2 = o.q2;
+        D.g2 = o.g2;
+        D.t2 = o.t2;
+        D.na = L.na;
+        D.nb = L.nb;
+        for (int h = 0; h < 8; h++) {
+            D.amopi[h] = o.amopi[h];
+            D.poff[h] = L.poff[h];
+            D.gems_ab[h] = L.gems_ab[h];
+            D.gems_aa[h] = L.gems_aa[h];
+            D.trip_aab[h] = L.trip_aab[h];
+            D.trip_aba[h] = L.trip_aba[h];
+            D.bab_off[h] = L.bab_off[h];
+            D.baa_off[h] = L.baa_off[h];
+            D.baab_off[h] = L.baab_off[h];
+            D.baba_off[h] = L.baba_off[h];
+        }
+        std::memcpy(D.xoff, L.xoff, sizeof(D.xoff));
+        std::memcpy(D.rowoff, L.rowoff, sizeof(D.rowoff));
+        D.sym = L.sym.data();
+        D.gab = L.gab.data();
+        D.gaa = L.gaa.data();
+        D.bab = L.bab.data();
+        D.baa = L.baa.data();
+        D.baab = L.baab.data();
+        D.baba = L.baba.data();
+    }
+};
+
+struct CheckedGather {
+    const double *x;
+    int64_t n;
+    long long *viol;
+    double acc = 0.0;
+    void operator()(int64_t col, double coef)
+    {
+        if (col < 0 || col >= n) { (*viol)++; return; }
+        acc += coef * x[col];
+    }
+};
+
+struct CheckedScatter {
+    double *out;
+    int64_t n;
+    long long *viol;
+    double y;
+    void operator()(int64_t col, double coef)
+    {
+        if (col < 0 || col >= n) { (*viol)++; return; }
+        out[col] += coef * y;
+    }
+};
+
+extern "C" {
+
+void *emu_create(int nirrep, const int *amopi, int na, int nb, int mult, int q2, int g2, int t2,
+                 int spin, int sz)
+{
+    Options o;
+    o.nirrep = nirrep;
+    for (int h = 0; h < 8; h++) o.amopi[h] = h < nirrep ? amopi[h] : 0;
+    o.nalpha = na;
+    o.nbeta = nb;
+    o.multiplicity = mult;
+    o.q2 = q2;
+    o.g2 = g2;
+    o.t2 = t2;
+    o.spin = spin;
+    o.sz = sz;
+    try {
+        return new Emu(o);
+    } catch (...) {
+        return nullptr;
+    }
+}
+
+void emu_destroy(void *h) { delete (Emu *)h; }
+
+void emu_dims(void *h, long long *dimx, long long *ncon)
+{
+    Emu *E = (Emu *)h;
+    *dimx = E->L.dimx;
+    *ncon = E->L.ncon;
+}
+
+long long emu_violations(void *h) { return ((Emu *)h)->violations; }
+
+// A.x with the device row code; also checks segments tile the rows exactly
+void emu_Au(void *h, const double *x, double *Ax)
+{
+    Emu *E = (Emu *)h;
+    const int64_t ncon = E->L.ncon;
+    std::vector<char> seen(ncon, 0);
+    for (const Segment &sg : E->L.rowsegs) {
+        for (int64_t e = 0; e < sg.count; e++) {
+            const int64_t r = sg.start + e;
+            if (r < 0 || r >= ncon || seen[r]) { E->violations++; continue; }
+            seen[r] = 1;
+            double v = 0.0;
+            if (sg.wave) {
+                for (int lane = 0; lane < 64; lane++) {
+                    CheckedGather f{x, E->L.dimx, &E->violations};
+                    rows::wave_terms(E->D, sg, e, lane, f);
+                    v += f.acc;
+                }
+            } else {
+                CheckedGather f{x, E->L.dimx, &E->violations};
+                rows::elem_terms(E->D, sg, e, f);
+                v = f.acc;
+            }
+            Ax[r] = v;
+        }
+    }
+    for (int64_t r = 0; r < ncon; r++)
+        if (!seen[r]) E->violations++;
+}
+
+// A^T.y with the device gathers; y must point into a buffer with ncon
+// NaN entries on each side
+void emu_ATu(void *h, const double *y, double *ATy)
+{
+    Emu *E = (Emu *)h;
+    const int64_t dimx = E->L.dimx;
+    std::vector<char> seen(dimx, 0);
+    for (const Segment &sg : E->L.xsegs) {
+        for (int64_t e = 0; e < sg.count; e++) {
+            const int64_t i = sg.start + e;
+            if (i < 0 || i >= dimx || seen[i]) { E->violations++; continue; }
+            seen[i] = 1;
+            double v = 0.0;
+            if (sg.wave) {
+                for (int lane = 0; lane < 64; lane++) v += rows::atu_d1_wave(E->D, sg, e, lane, y);
+            } else if (sg.kind == XS_D2AB) {
+                v = rows::atu_d2ab(E->D, sg, e, y);
+            } else if (sg.kind == XS_D2AA) {
+                v = rows::atu_d2aa(E->D, sg, e, y);
+            } else {
+                v = rows::atu_self(E->D, sg, e, y);
+            }
+            ATy[i] = v;
+        }
+    }
+    for (const Segment &sg : E->L.xsegs_t2) {
+        for (int64_t e = 0; e < sg.count; e++) {
+            const int64_t i = sg.start + e;
+            if (i < 0 || i >= dimx || seen[i]) { E->violations++; continue; }
+            seen[i] = 1;
+            const double yv = y[sg.aux + e];
+            ATy[i] = -yv;
+            CheckedScatter f{ATy, dimx, &E->violations, yv};
+            rows::t2_terms(E->D, sg.kind, sg.sub, sg.h, e, f, false);
+        }
+    }
+    for (int64_t i = 0; i < dimx; i++)
+        if (!seen[i]) E->violations++;
+}
+
+void emu_row_norm2(void *h, double *d)
+{
+    Emu *E = (Emu *)h;
+    for (const Segment &sg : E->L.rowsegs) {
+        for (int64_t e = 0; e < sg.count; e++) {
+            double v = 0.0;
+            if (sg.wave) {
+                if (sg.kind == RF_TRACE) {
+                    const double a = E->L.amo, npair = a * (a - 1.0) / 2.0;
+                    if (sg.sub == 0) v = a * a;
+                    else if (sg.sub < 3) v = 4.0 * npair;
+                    else v = 4.0 * a * a + 2.0 * 4.0 * npair;
+                } else {
+                    for (int lane = 0; lane < 64; lane++) {
+                        rows::SumSq f;
+                        rows::wave_terms(E->D, sg, e, lane, f);
+                        v += f.acc;
+                    }
+                }
+            } else {
+                rows::Collect f;
+                rows::elem_terms(E->D, sg, e, f);
+                v = f.norm2();
+            }
+            d[sg.start + e] = v;
+        }
+    }
+}
+
+}  // extern "C"

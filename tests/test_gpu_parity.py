@@ -1,0 +1,128 @@
+"""Parity of the HIP engine (through the C ABI) with the CPU oracle.
+
+Tolerances (FP64): per-call maps 1e-12 normwise relative; CG solution and
+Update_xz 1e-9 relative (different but equally valid summation orders and
+eigensolvers); converged BPSDP energies 1e-6 Eh at R/E_CONVERGENCE 1e-6.
+"""
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+import oracle_py as orc  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+LAYOUTS = [
+    # amopi, nalpha, nbeta, mult, positivity, sz
+    ([4], 2, 2, 1, "DQG", True),
+    ([5], 3, 2, 2, "DQG", True),
+    ([6], 4, 2, 3, "DQG", True),
+    ([2, 1, 1, 0], 2, 2, 1, "DQG", True),
+    ([2, 1, 1, 1, 0, 1, 1, 1], 3, 3, 1, "DQG", True),
+    ([4], 2, 2, 1, "DQG", False),
+    ([5], 2, 2, 1, "DQ", True),
+    ([5], 2, 2, 1, "DG", True),
+    ([5], 3, 2, 2, "DQGT2", True),
+    ([2, 1, 1, 1], 2, 2, 1, "DQGT2", True),
+    ([9], 4, 4, 1, "DQG", True),
+    ([5, 0, 0, 0, 0, 5, 0, 0], 5, 5, 1, "DQG", True),
+]
+
+
+def _pair(amopi, na, nb, mult, pos, sz):
+    from v2rdm_casscf_amd import V2RDMSolver
+    s = V2RDMSolver(amopi, na, nb, mult, pos, constrain_sz=sz)
+    L = orc.Layout(amopi, na, nb, mult, pos, constrain_sz=sz)
+    assert (s.dimx, s.nconstraints) == (L.dimx, L.ncon)
+    assert s.dimensions == L.block_dims
+    return s, L
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300)
+
+
+def _dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+@pytest.mark.parametrize("amopi,na,nb,mult,pos,sz", LAYOUTS)
+def test_maps_match_oracle(amopi, na, nb, mult, pos, sz):
+    s, L = _pair(amopi, na, nb, mult, pos, sz)
+    rng = np.random.default_rng(5)
+    x = rng.normal(size=L.dimx)
+    y = rng.normal(size=L.ncon)
+    ax = s.bpsdp_Au(torch.empty(L.ncon, dtype=torch.float64, device="cuda"), _dev(x))
+    aty = s.bpsdp_ATu(torch.empty(L.dimx, dtype=torch.float64, device="cuda"), _dev(y))
+    ref_ax, ref_aty = L.Au(x), L.ATu(y)
+    fam = {}
+    for name, st, cnt in L.row_families():
+        e = _rel(ax.cpu().numpy()[st:st + cnt], ref_ax[st:st + cnt])
+        fam[name] = e
+    assert _rel(ax.cpu().numpy(), ref_ax) < 1e-12, fam
+    assert _rel(aty.cpu().numpy(), ref_aty) < 1e-12
+    u = rng.normal(size=L.ncon)
+    out = s.cg_Ax(L.ncon, torch.empty(L.ncon, dtype=torch.float64, device="cuda"), _dev(u))
+    assert _rel(out.cpu().numpy(), L.cg_Ax(u)) < 1e-12
+
+
+@pytest.mark.parametrize("amopi,na,nb,mult,pos,sz", LAYOUTS[:6])
+def test_row_norm2_matches_dense(amopi, na, nb, mult, pos, sz):
+    s, L = _pair(amopi, na, nb, mult, pos, sz)
+    A = np.stack([L.Au(e) for e in np.eye(L.dimx)], axis=1)
+    d = s.row_norm2().cpu().numpy()
+    assert np.allclose(d, (A * A).sum(axis=1), rtol=1e-13, atol=1e-13)
+
+
+@pytest.mark.parametrize("amopi,na,nb,mult,pos,sz", [LAYOUTS[0], LAYOUTS[3], LAYOUTS[10]])
+def test_cg_matches_oracle(amopi, na, nb, mult, pos, sz):
+    s, L = _pair(amopi, na, nb, mult, pos, sz)
+    rng = np.random.default_rng(6)
+    rhs = L.Au(rng.normal(size=L.dimx))
+    y0 = rng.normal(size=L.ncon)
+    yref, itref = L.cg_solve(rhs, y0, 1e-8, 500)
+    y = _dev(y0.copy())
+    it, res = s.cg_solve(_dev(rhs), y, 1e-8, 500)
+    assert abs(it - itref) <= 1
+    # A^T y is what the iteration uses; y itself is unique only up to null(A^T)
+    assert _rel(L.ATu(y.cpu().numpy()), L.ATu(yref)) < 1e-7
+    assert res < 1e-8
+
+
+@pytest.mark.parametrize("amopi,na,nb,mult,pos,sz", [LAYOUTS[0], LAYOUTS[3], LAYOUTS[8],
+                                                     LAYOUTS[10], LAYOUTS[11]])
+def test_update_xz_matches_oracle(amopi, na, nb, mult, pos, sz):
+    s, L = _pair(amopi, na, nb, mult, pos, sz)
+    rng = np.random.default_rng(7)
+    y = rng.normal(size=L.ncon)
+    c = rng.normal(size=L.dimx)
+    x0 = rng.normal(size=L.dimx)
+    xr, zr = L.update_xz(0.8, y, c, x0)
+    x, z = _dev(x0.copy()), _dev(np.zeros(L.dimx))
+    s.update_xz(0.8, _dev(y), _dev(c), x, z)
+    assert _rel(x.cpu().numpy(), xr) < 1e-10
+    assert _rel(z.cpu().numpy(), zr) < 1e-10
+    # second call runs warm from the kept eigenvectors
+    y2 = y + 1e-3 * rng.normal(size=L.ncon)
+    xr2, zr2 = L.update_xz(0.8, y2, c, xr)
+    s.update_xz(0.8, _dev(y2), _dev(c), x, z)
+    assert _rel(x.cpu().numpy(), xr2) < 1e-10
+    assert _rel(z.cpu().numpy(), zr2) < 1e-10
+
+
+@pytest.mark.parametrize("amopi,na,nb,pos", [([4], 2, 2, "DQG"), ([2, 1, 1, 0], 2, 2, "DQG"),
+                                             ([5], 3, 2, "DQG")])
+def test_bpsdp_energy_matches_oracle(amopi, na, nb, pos):
+    from v2rdm_casscf_amd import V2RDMSolver
+    import fci_tools as ft
+    L = orc.Layout(amopi, na, nb, None, pos)
+    irreps = [L.irrep(p) for p in range(L.amo)]
+    h1, eri = ft.random_hamiltonian(L.amo, 0, irreps if len(amopi) > 1 else None)
+    c, b = L.build_c(h1, eri), L.build_b()
+    x, z, y = L.guess_random()
+    ref = L.bpsdp(c, b, x, z, y, r_conv=1e-6, e_conv=1e-6)
+    s = V2RDMSolver(amopi, na, nb, None, pos)
+    out = s.compute_energy(h1, eri)
+    assert out["converged"] and ref["converged"]
+    assert abs(out["energy_primal"] - ref["energy_primal"]) < 1e-6

@@ -1,0 +1,672 @@
+// api.hip -- the v2g C ABI (include/v2g.h): solver handle, device buffers,
+// the CG driver, the Update_xz driver and the BPSDP outer loop.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/v2g.h"
+#include "devlayout.hpp"
+#include "eig.hpp"
+#include "kernels.hpp"
+#include "layout.hpp"
+
+namespace {
+
+thread_local std::string g_err;
+
+struct HipError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+#define HIPCHK(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            throw HipError(std::string(#expr) + ": " + hipGetErrorString(e_));              \
+    } while (0)
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+    void alloc(size_t count)
+    {
+        if (p) { (void)hipFree(p); p = nullptr; }
+        n = count;
+        if (count) HIPCHK(hipMalloc(&p, sizeof(T) * count));
+    }
+    void upload(const std::vector<T> &v)
+    {
+        alloc(v.size());
+        if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
+    }
+};
+
+double now()
+{
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+}  // namespace
+
+struct v2g_solver {
+    std::unique_ptr<v2g::Layout> L;
+    hipStream_t own = nullptr, st = nullptr;
+    // layout on device
+    DevBuf<int> d_sym, d_gab, d_gaa, d_bab, d_baa, d_baab, d_baba;
+    DevBuf<v2g::DevLayout> d_lay;
+    DevBuf<v2g::Segment> d_rowsegs, d_xsegs, d_t2segs;
+    DevBuf<int64_t> d_rowtiles, d_xtiles, d_t2tiles;
+    v2g::MapPlan plan;
+    // CG
+    DevBuf<double> r, p, z, Ap, t, partial, scal, jacobi, user_minv;
+    bool have_jacobi = false, have_user = false;
+    // loop scratch
+    DevBuf<double> Ax, Bv, ATy, red;
+    // eigen split
+    std::vector<v2g::EigBlock> small, large;
+    DevBuf<v2g::EigBlock> d_small, d_large;
+    DevBuf<double> Mw, Aw, Tw, Vw, Qbuf, tol;
+    DevBuf<int> active, counter;
+    DevBuf<v2g::GemmDesc> d_gemm_warm1, d_gemm_warm2, d_gemm_rec;
+    int max_pairs = 0, max_nb = 0, max_N = 0;
+    int64_t max_tiles = 0;
+    bool warm = false;
+    int last_sweeps = 0;
+    v2g_iter_callback cb = nullptr;
+    void *cb_user = nullptr;
+
+    explicit v2g_solver(const v2g_layout_desc &d);
+    ~v2g_solver()
+    {
+        if (own) (void)hipStreamDestroy(own);
+    }
+    void ensure_jacobi();
+    int cg(const double *rhs, double *y, double tol, int maxiter, int precond, int flexible,
+           double *res);
+    void update_xz(double mu, const double *y, const double *c, double *x, double *z);
+    double dot(const double *a, const double *b, int64_t n);
+};
+
+v2g_solver::v2g_solver(const v2g_layout_desc &d)
+{
+    v2g::Options o;
+    o.nirrep = d.nirrep;
+    for (int h = 0; h < 8; h++) o.amopi[h] = h < d.nirrep ? d.amopi[h] : 0;
+    o.nalpha = d.nalpha;
+    o.nbeta = d.nbeta;
+    o.multiplicity = d.multiplicity;
+    o.q2 = d.constrain_q2;
+    o.g2 = d.constrain_g2;
+    o.t2 = d.constrain_t2;
+    o.spin = d.constrain_spin;
+    o.sz = d.constrain_sz;
+    L.reset(new v2g::Layout(o));
+    HIPCHK(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
+    st = own;
+
+    d_sym.upload(L->sym.empty() ? std::vector<int>{0} : L->sym);
+    d_gab.upload(L->gab.empty() ? std::vector<int>{0} : L->gab);
+    d_gaa.upload(L->gaa.empty() ? std::vector<int>{0} : L->gaa);
+    d_bab.upload(L->bab);
+    d_baa.upload(L->baa);
+    d_baab.upload(L->baab);
+    d_baba.upload(L->baba);
+    v2g::DevLayout dl{};
+    dl.nirrep = L->nirrep;
+    dl.amo = L->amo;
+    dl.singlet = L->singlet;
+    dl.sz = o.sz;
+    dl.spin = o.spin;
+    dl.q2 = o.q2;
+    dl.g2 = o.g2;
+    dl.t2 = o.t2;
+    dl.na = L->na;
+    dl.nb = L->nb;
+    for (int h = 0; h < 8; h++) {
+        dl.amopi[h] = o.amopi[h];
+        dl.poff[h] = L->poff[h];
+        dl.gems_ab[h] = L->gems_ab[h];
+        dl.gems_aa[h] = L->gems_aa[h];
+        dl.trip_aab[h] = L->trip_aab[h];
+        dl.trip_aba[h] = L->trip_aba[h];
+        dl.bab_off[h] = L->bab_off[h];
+        dl.baa_off[h] = L->baa_off[h];
+        dl.baab_off[h] = L->baab_off[h];
+        dl.baba_off[h] = L->baba_off[h];
+    }
+    std::memcpy(dl.xoff, L->xoff, sizeof(dl.xoff));
+    std::memcpy(dl.rowoff, L->rowoff, sizeof(dl.rowoff));
+    dl.sym = d_sym.p;
+    dl.gab = d_gab.p;
+    dl.gaa = d_gaa.p;
+    dl.bab = d_bab.p;
+    dl.baa = d_baa.p;
+    dl.baab = d_baab.p;
+    dl.baba = d_baba.p;
+    d_lay.upload(std::vector<v2g::DevLayout>{dl});
+
+    auto tiles = [](const std::vector<v2g::Segment> &segs, std::vector<int64_t> &ts) {
+        ts.assign(1, 0);
+        for (const auto &s : segs) ts.push_back(ts.back() + v2g::tiles_for(s));
+        return ts.back();
+    };
+    std::vector<int64_t> ts;
+    plan.dlay = d_lay.p;
+    plan.row_tiles = tiles(L->rowsegs, ts);
+    d_rowsegs.upload(L->rowsegs);
+    d_rowtiles.upload(ts);
+    plan.rowsegs = d_rowsegs.p;
+    plan.row_tile_start = d_rowtiles.p;
+    plan.nrowsegs = (int)L->rowsegs.size();
+    plan.x_tiles = tiles(L->xsegs, ts);
+    d_xsegs.upload(L->xsegs);
+    d_xtiles.upload(ts);
+    plan.xsegs = d_xsegs.p;
+    plan.x_tile_start = d_xtiles.p;
+    plan.nxsegs = (int)L->xsegs.size();
+    plan.t2_tiles = tiles(L->xsegs_t2, ts);
+    d_t2segs.upload(L->xsegs_t2);
+    d_t2tiles.upload(ts);
+    plan.t2segs = d_t2segs.p;
+    plan.t2_tile_start = d_t2tiles.p;
+    plan.nt2segs = (int)L->xsegs_t2.size();
+
+    const int64_t ncon = L->ncon, dimx = L->dimx;
+    r.alloc(ncon);
+    p.alloc(ncon);
+    z.alloc(ncon);
+    Ap.alloc(ncon);
+    t.alloc(dimx);
+    partial.alloc(v2g::kPartialArrays * v2g::kRedBlocks);
+    scal.alloc(v2g::CG_NSLOTS);
+    Ax.alloc(ncon);
+    Bv.alloc(ncon);
+    ATy.alloc(dimx);
+    red.alloc(v2g::kRedBlocks + 8);
+
+    // eigen-split plan: blocks <= 64 whole in one workgroup, others block Jacobi
+    int64_t woff = 0, qoff = 0, poff = 0;
+    for (size_t b = 0; b < L->block_dims.size(); b++) {
+        const int n = L->block_dims[b];
+        if (n == 0) continue;
+        v2g::EigBlock e{};
+        e.n = n;
+        e.xoff = L->block_offs[b];
+        if (n <= 64) {
+            e.N = 64;
+            small.push_back(e);
+            continue;
+        }
+        e.N = ((n + 63) / 64) * 64;
+        e.woff = woff;
+        e.qoff = qoff;
+        e.poff = poff;
+        const int np = e.N / 64;
+        woff += (int64_t)e.N * e.N;
+        qoff += (int64_t)np * 64 * 64;
+        poff += np;
+        max_pairs = std::max(max_pairs, np);
+        max_nb = std::max(max_nb, e.N / 32);
+        max_N = std::max(max_N, e.N);
+        max_tiles = std::max(max_tiles, 2 * (int64_t)np * np);
+        large.push_back(e);
+    }
+    d_small.upload(small);
+    d_large.upload(large);
+    if (!large.empty()) {
+        Mw.alloc(woff);
+        Aw.alloc(woff);
+        Tw.alloc(woff);
+        Vw.alloc(woff);
+        Qbuf.alloc(qoff);
+        active.alloc(poff);
+        tol.alloc(large.size());
+        counter.alloc(1);
+    }
+}
+
+void v2g_solver::ensure_jacobi()
+{
+    if (have_jacobi) return;
+    jacobi.alloc(L->ncon);
+    v2g::launch_row_norm2(plan, jacobi.p, st);
+    std::vector<double> d(L->ncon);
+    HIPCHK(hipMemcpyAsync(d.data(), jacobi.p, sizeof(double) * d.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (auto &v : d) v = v > 0.0 ? 1.0 / v : 1.0;
+    HIPCHK(hipMemcpyAsync(jacobi.p, d.data(), sizeof(double) * d.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    have_jacobi = true;
+}
+
+double v2g_solver::dot(const double *a, const double *b, int64_t n)
+{
+    v2g::vec_dot_partial(n, a, b, red.p, st);
+    v2g::vec_reduce(red.p, red.p + v2g::kRedBlocks, st);
+    double out = 0.0;
+    HIPCHK(hipMemcpyAsync(&out, red.p + v2g::kRedBlocks, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return out;
+}
+
+int v2g_solver::cg(const double *rhs, double *y, double tolv, int maxiter, int precond,
+                   int flexible, double *res)
+{
+    const int64_t n = L->ncon;
+    const double *minv = nullptr;
+    if (precond == 1) { ensure_jacobi(); minv = jacobi.p; }
+    else if (precond == 2) {
+        if (!have_user) throw std::invalid_argument("precond 2 needs v2g_set_precond_diag");
+        minv = user_minv.p;
+    }
+    double *zz = minv ? z.p : nullptr;
+    // Ap = A A^T y ; r = b - Ap ; p = M^-1 r   (cg_solver.cc:141-150)
+    v2g::launch_atu(plan, y, t.p, st);
+    v2g::launch_au(plan, t.p, Ap.p, st);
+    v2g::cg_init(n, rhs, Ap.p, r.p, p.p, minv, zz, partial.p, scal.p, tolv, maxiter, st);
+    const int chunk = 8;
+    double hs[v2g::CG_NSLOTS];
+    int it = 0;
+    for (;;) {
+        for (int c = 0; c < chunk; c++, it++) {
+            const int par = it & 1;
+            const double *done = scal.p + v2g::CG_DONE0 + par;
+            v2g::launch_atu(plan, p.p, t.p, st, done);
+            v2g::launch_au(plan, t.p, Ap.p, st, done);
+            v2g::cg_dot(n, p.p, Ap.p, partial.p, scal.p, par, st);
+            v2g::cg_step(n, y, r.p, p.p, Ap.p, minv, zz, nullptr, partial.p, scal.p, par, st);
+            v2g::cg_finish_iter(n, r.p, p.p, zz, partial.p, scal.p, par, flexible, it, st);
+        }
+        HIPCHK(hipMemcpyAsync(hs, scal.p, sizeof(hs), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (hs[v2g::CG_DONE0 + (it & 1)] != 0.0) break;
+        if (it > maxiter + 2 * chunk) break;  // defensive: cannot happen
+    }
+    if (res) *res = hs[v2g::CG_NRM];
+    return (int)hs[v2g::CG_ITER];
+}
+
+void v2g_solver::update_xz(double mu, const double *y, const double *c, double *x, double *zv)
+{
+    v2g::launch_atu(plan, y, ATy.p, st);
+    v2g::eig_small(d_small.p, (int)small.size(), ATy.p, c, x, zv, mu, st);
+    last_sweeps = 0;
+    if (large.empty()) return;
+    const int nl = (int)large.size();
+    const int cold = warm ? 0 : 1;
+    v2g::eig_build_m(d_large.p, nl, ATy.p, c, x, mu, Mw.p, Aw.p, Vw.p, cold, st);
+    v2g::eig_tol(d_large.p, nl, Mw.p, tol.p, st);
+    if (!cold) {
+        // A = V^T (M V)
+        if (!d_gemm_warm1.p) {
+            std::vector<v2g::GemmDesc> g1, g2;
+            for (const auto &b : large) {
+                g1.push_back({Mw.p + b.woff, Vw.p + b.woff, Tw.p + b.woff, b.N, b.N, b.N, b.N, b.N,
+                              b.N, 0, 0});
+                g2.push_back({Vw.p + b.woff, Tw.p + b.woff, Aw.p + b.woff, b.N, b.N, b.N, b.N, b.N,
+                              b.N, 1, 0});
+            }
+            d_gemm_warm1.upload(g1);
+            d_gemm_warm2.upload(g2);
+        }
+        v2g::gemm_batched(d_gemm_warm1.p, nl, max_N, max_N, st);
+        v2g::gemm_batched(d_gemm_warm2.p, nl, max_N, max_N, st);
+    }
+    const int maxsweeps = 40;
+    int sweeps = 0;
+    for (; sweeps < maxsweeps; sweeps++) {
+        HIPCHK(hipMemsetAsync(counter.p, 0, sizeof(int), st));
+        for (int rnd = 0; rnd < max_nb - 1; rnd++)
+            v2g::eig_round(d_large.p, nl, max_pairs, max_tiles, rnd, Aw.p, Vw.p, Qbuf.p, active.p,
+                           tol.p, counter.p, st);
+        int cnt = 0;
+        HIPCHK(hipMemcpyAsync(&cnt, counter.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipStreamSynchronize(st));
+        if (cnt == 0) break;
+    }
+    last_sweeps = sweeps;
+    // x = (V s)(V s)^T with s = sqrt(max(lambda,0)/mu); z = mu x - M
+    v2g::eig_scale(d_large.p, nl, Aw.p, Vw.p, Tw.p, mu, st);
+    std::vector<v2g::GemmDesc> g3;
+    int maxn = 0;
+    for (const auto &b : large) {
+        g3.push_back({Tw.p + b.woff, Tw.p + b.woff, x + b.xoff, b.n, b.n, b.N, b.N, b.N, b.n, 0, 1});
+        maxn = std::max(maxn, b.n);
+    }
+    d_gemm_rec.upload(g3);
+    v2g::gemm_batched(d_gemm_rec.p, nl, maxn, maxn, st);
+    v2g::eig_z(d_large.p, nl, x, Mw.p, zv, mu, st);
+    warm = true;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+#define API_BEGIN try {
+#define API_END                                                                           \
+    return 0;                                                                             \
+    }                                                                                     \
+    catch (const std::invalid_argument &e) { g_err = e.what(); return -1; }              \
+    catch (const HipError &e) { g_err = e.what(); return -2; }                           \
+    catch (const std::exception &e) { g_err = e.what(); return -3; }
+
+extern "C" {
+
+int v2g_create(const v2g_layout_desc *desc, v2g_handle *out)
+{
+    API_BEGIN
+    if (!desc || !out) throw std::invalid_argument("null argument");
+    *out = new v2g_solver(*desc);
+    API_END
+}
+
+int v2g_destroy(v2g_handle h)
+{
+    API_BEGIN
+    delete h;
+    API_END
+}
+
+int v2g_dims(v2g_handle h, long long *dimx, long long *ncon, int *nblocks)
+{
+    API_BEGIN
+    if (!h) throw std::invalid_argument("null handle");
+    if (dimx) *dimx = h->L->dimx;
+    if (ncon) *ncon = h->L->ncon;
+    if (nblocks) *nblocks = (int)h->L->block_dims.size();
+    API_END
+}
+
+int v2g_block_dims(v2g_handle h, int *dims)
+{
+    API_BEGIN
+    if (!h || !dims) throw std::invalid_argument("null argument");
+    std::copy(h->L->block_dims.begin(), h->L->block_dims.end(), dims);
+    API_END
+}
+
+int v2g_set_stream(v2g_handle h, void *s)
+{
+    API_BEGIN
+    if (!h) throw std::invalid_argument("null handle");
+    h->st = s ? (hipStream_t)s : h->own;
+    API_END
+}
+
+int v2g_Au(v2g_handle h, const double *x, double *Ax)
+{
+    API_BEGIN
+    if (!h || !x || !Ax) throw std::invalid_argument("null argument");
+    v2g::launch_au(h->plan, x, Ax, h->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->st));
+    API_END
+}
+
+int v2g_ATu(v2g_handle h, const double *y, double *ATy)
+{
+    API_BEGIN
+    if (!h || !y || !ATy) throw std::invalid_argument("null argument");
+    v2g::launch_atu(h->plan, y, ATy, h->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->st));
+    API_END
+}
+
+int v2g_cg_Ax(v2g_handle h, const double *u, double *out)
+{
+    API_BEGIN
+    if (!h || !u || !out) throw std::invalid_argument("null argument");
+    v2g::launch_atu(h->plan, u, h->t.p, h->st);
+    v2g::launch_au(h->plan, h->t.p, out, h->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->st));
+    API_END
+}
+
+int v2g_row_norm2(v2g_handle h, double *d)
+{
+    API_BEGIN
+    if (!h || !d) throw std::invalid_argument("null argument");
+    v2g::launch_row_norm2(h->plan, d, h->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->st));
+    API_END
+}
+
+int v2g_cg_solve(v2g_handle h, const double *rhs, double *y, double tol, int maxiter, int precond,
+                 int flexible, int *iters, double *resnorm)
+{
+    API_BEGIN
+    if (!h || !rhs || !y) throw std::invalid_argument("null argument");
+    if (precond < 0 || precond > 2) throw std::invalid_argument("precond must be 0, 1 or 2");
+    const int it = h->cg(rhs, y, tol, maxiter, precond, flexible, resnorm);
+    if (iters) *iters = it;
+    HIPCHK(hipGetLastError());
+    API_END
+}
+
+int v2g_set_precond_diag(v2g_handle h, const double *minv)
+{
+    API_BEGIN
+    if (!h || !minv) throw std::invalid_argument("null argument");
+    if (!h->user_minv.p) h->user_minv.alloc(h->L->ncon);
+    HIPCHK(hipMemcpyAsync(h->user_minv.p, minv, sizeof(double) * h->L->ncon,
+                          hipMemcpyDeviceToDevice, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    h->have_user = true;
+    API_END
+}
+
+int v2g_update_xz(v2g_handle h, double mu, const double *y, const double *c, double *x, double *z)
+{
+    API_BEGIN
+    if (!h || !y || !c || !x || !z) throw std::invalid_argument("null argument");
+    if (!(mu > 0.0)) throw std::invalid_argument("mu must be positive");
+    h->update_xz(mu, y, c, x, z);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->st));
+    API_END
+}
+
+int v2g_reset_eig(v2g_handle h)
+{
+    API_BEGIN
+    if (!h) throw std::invalid_argument("null handle");
+    h->warm = false;
+    API_END
+}
+
+int v2g_set_iter_callback(v2g_handle h, v2g_iter_callback cb, void *user)
+{
+    API_BEGIN
+    if (!h) throw std::invalid_argument("null handle");
+    h->cb = cb;
+    h->cb_user = user;
+    API_END
+}
+
+int v2g_bpsdp(v2g_handle h, const v2g_params *prm, const double *c, const double *b, double *x,
+              double *z, double *y, v2g_stats *out)
+{
+    API_BEGIN
+    if (!h || !prm || !c || !b || !x || !z || !y) throw std::invalid_argument("null argument");
+    const int64_t ncon = h->L->ncon, dimx = h->L->dimx;
+    hipStream_t st = h->st;
+    const double tau = prm->tau;
+    double mu = 1.0, ep = 0.0, ed = 0.0, egap = 0.0, ecur = 0.0, edual = 0.0;
+    int oiter = 0;
+    long long cgtot = 0;
+    double tcg = 0.0, txz = 0.0;
+    const double t0 = now();
+    v2g_stats s{};
+    bool stop = false;
+    do {
+        const double t1 = now();
+        // Ax = -tau mu (A x - b); z = c - z; B = A z + Ax  (v2rdm_solver.cc:1784-1794)
+        v2g::launch_au(h->plan, x, h->Ax.p, st);
+        v2g::vec_axpby(ncon, -tau * mu, h->Ax.p, tau * mu, b, h->Ax.p, st);
+        v2g::vec_axpby(dimx, 1.0, c, -1.0, z, z, st);
+        v2g::launch_au(h->plan, z, h->Bv.p, st);
+        v2g::vec_axpby(ncon, 1.0, h->Bv.p, 1.0, h->Ax.p, h->Bv.p, st);
+        double cgtol;  // v2rdm_solver.cc:1797-1805
+        if (oiter == 0) cgtol = 0.01;
+        else cgtol = (ep > ed) ? 0.01 * ed : 0.01 * ep;
+        if (cgtol < prm->cg_convergence) cgtol = prm->cg_convergence;
+        double res = 0.0;
+        const int iiter = h->cg(h->Bv.p, y, cgtol, prm->cg_maxiter, prm->precond, prm->flexible, &res);
+        cgtot += iiter;
+        const double t2 = now();
+        tcg += t2 - t1;
+        h->update_xz(mu, y, c, x, z);
+        HIPCHK(hipStreamSynchronize(st));
+        txz += now() - t2;
+        // ed = ||A^T y - c + z||, ep = ||A x - b||  (v2rdm_solver.cc:1829-1837)
+        v2g::launch_atu(h->plan, y, h->ATy.p, st);
+        v2g::vec_axpby(dimx, 1.0, h->ATy.p, 1.0, z, h->ATy.p, st);
+        v2g::vec_axpby(dimx, 1.0, h->ATy.p, -1.0, c, h->ATy.p, st);
+        ed = std::sqrt(h->dot(h->ATy.p, h->ATy.p, dimx));
+        v2g::launch_au(h->plan, x, h->Ax.p, st);
+        v2g::vec_axpby(ncon, 1.0, h->Ax.p, -1.0, b, h->Ax.p, st);
+        ep = std::sqrt(h->dot(h->Ax.p, h->Ax.p, ncon));
+        if (oiter % prm->mu_update_frequency == 0 && oiter > 0) mu = mu * ep / ed;
+        ecur = h->dot(c, x, dimx);
+        edual = h->dot(b, y, ncon);
+        if (prm->verbose)
+            std::fprintf(stderr, "      %5i %5i %14.8f %14.8f %11.4e %7.3f %10.5e %10.5e  sweeps %d\n",
+                         oiter, iiter, ecur, edual, std::fabs(ecur - edual), mu, ep, ed,
+                         h->last_sweeps);
+        oiter++;
+        egap = std::fabs(ecur - edual);
+        s.oiter = oiter;
+        s.cg_iters = cgtot;
+        s.energy_primal = ecur;
+        s.energy_dual = edual;
+        s.ep = ep;
+        s.ed = ed;
+        s.mu = mu;
+        s.t_cg = tcg;
+        s.t_xz = txz;
+        s.t_total = now() - t0;
+        s.eig_sweeps = h->last_sweeps;
+        if (h->cb && h->cb(h->cb_user, oiter, &s)) stop = true;
+        if (oiter == prm->maxiter) break;
+        if (prm->max_seconds > 0 && now() - t0 > prm->max_seconds) break;
+    } while (!stop && (ep > prm->r_convergence || ed > prm->r_convergence || egap > prm->e_convergence));
+    s.converged = !(ep > prm->r_convergence || ed > prm->r_convergence || egap > prm->e_convergence);
+    s.t_total = now() - t0;
+    if (out) *out = s;
+    HIPCHK(hipGetLastError());
+    API_END
+}
+
+int v2g_build_b(v2g_handle h, double *b)
+{
+    API_BEGIN
+    if (!h || !b) throw std::invalid_argument("null argument");
+    const v2g::Layout &L = *h->L;
+    std::fill(b, b + L.ncon, 0.0);
+    const double na = L.na, nb = L.nb, n = na + nb;
+    if (L.opt.sz) {
+        b[L.rowoff[v2g::RF_TRACE][0][0]] = na * nb;
+        b[L.rowoff[v2g::RF_TRACE][1][0]] = na * (na - 1.0);
+        b[L.rowoff[v2g::RF_TRACE][2][0]] = nb * (nb - 1.0);
+    } else {
+        b[L.rowoff[v2g::RF_TRACE][3][0]] = n * (n - 1.0);
+    }
+    for (int s = 0; s < 2; s++)
+        for (int hh = 0; hh < L.nirrep; hh++) {
+            const int m = L.opt.amopi[hh];
+            const int64_t o = L.rowoff[v2g::RF_D1Q1][s][hh];
+            for (int i = 0; i < m; i++) b[o + (int64_t)i * m + i] = 1.0;
+        }
+    if (L.opt.spin) {
+        const double ms = (L.opt.multiplicity - 1.0) / 2.0;
+        b[L.rowoff[v2g::RF_S2][0][0]] = 0.5 * (na + nb) + ms * ms - ms * (ms + 1.0);
+    }
+    if (L.opt.q2) {
+        for (int hh = 0; hh < L.nirrep; hh++) {
+            const int64_t g = L.gems_ab[hh], o = L.rowoff[v2g::RF_Q2AB][0][hh];
+            for (int64_t ij = 0; ij < g; ij++) b[o + ij * g + ij] = -1.0;
+        }
+        for (int s = 0; s < 2; s++)
+            for (int hh = 0; hh < L.nirrep; hh++) {
+                const int64_t g = L.gems_aa[hh], o = L.rowoff[v2g::RF_Q2AA][s][hh];
+                for (int64_t ij = 0; ij < g; ij++) b[o + ij * g + ij] = -1.0;
+            }
+    }
+    API_END
+}
+
+int v2g_build_c(v2g_handle h, const double *h1, const double *eri, double *c)
+{
+    API_BEGIN
+    if (!h || !h1 || !eri || !c) throw std::invalid_argument("null argument");
+    const v2g::Layout &L = *h->L;
+    const int64_t a = L.amo;
+    std::fill(c, c + L.dimx, 0.0);
+    auto E = [&](int p, int q, int r, int s) { return eri[((p * a + q) * a + r) * a + s]; };
+    for (int hh = 0; hh < L.nirrep; hh++) {
+        const int64_t g = L.gems_ab[hh];
+        const int *bp = L.bab.data() + L.bab_off[hh];
+        for (int64_t ij = 0; ij < g; ij++) {
+            const int i = bp[ij] & 0xffff, j = bp[ij] >> 16;
+            for (int64_t kl = 0; kl < g; kl++) {
+                const int k = bp[kl] & 0xffff, l = bp[kl] >> 16;
+                c[L.xoff[v2g::XB_D2AB][hh] + ij * g + kl] = E(i, k, j, l);
+            }
+        }
+        const int64_t ga = L.gems_aa[hh];
+        const int *ap = L.baa.data() + L.baa_off[hh];
+        for (int64_t ij = 0; ij < ga; ij++) {
+            const int i = ap[ij] & 0xffff, j = ap[ij] >> 16;
+            for (int64_t kl = 0; kl < ga; kl++) {
+                const int k = ap[kl] & 0xffff, l = ap[kl] >> 16;
+                const double v = E(i, k, j, l) - E(i, l, j, k);
+                c[L.xoff[v2g::XB_D2AA][hh] + ij * ga + kl] = v;
+                c[L.xoff[v2g::XB_D2BB][hh] + ij * ga + kl] = v;
+            }
+        }
+        const int m = L.opt.amopi[hh];
+        for (int i = 0; i < m; i++)
+            for (int j = 0; j < m; j++) {
+                const double v = h1[(int64_t)(i + L.poff[hh]) * a + (j + L.poff[hh])];
+                c[L.xoff[v2g::XB_D1A][hh] + (int64_t)i * m + j] = v;
+                c[L.xoff[v2g::XB_D1B][hh] + (int64_t)i * m + j] = v;
+            }
+    }
+    API_END
+}
+
+int v2g_guess_random(v2g_handle h, double *x, double *z, double *y)
+{
+    API_BEGIN
+    if (!h || !x || !z || !y) throw std::invalid_argument("null argument");
+    const v2g::Layout &L = *h->L;
+    srand(0);
+    for (int64_t i = 0; i < L.dimx; i++) {
+        x[i] = ((double)rand() / RAND_MAX - 1.0) * 2.0;
+        z[i] = ((double)rand() / RAND_MAX - 1.0) * 2.0;
+    }
+    for (int64_t i = 0; i < L.ncon; i++) y[i] = ((double)rand() / RAND_MAX - 1.0) * 2.0;
+    API_END
+}
+
+const char *v2g_last_error(void) { return g_err.c_str(); }
+const char *v2g_version(void) { return "v2g 0.1 (gfx950)"; }
+
+}  // extern "C"

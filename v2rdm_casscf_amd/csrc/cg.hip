@@ -1,0 +1,265 @@
+// cg.hip -- device-resident conjugate gradient on A A^T y = rhs.
+//
+// Same recurrence and stopping rule as CGSolver::solve (cg_solver.cc:125-177):
+// warm start from y, absolute ||r||_2 < tol tested after the update, iteration
+// count incremented after the p update, at most maxiter iterations.  Options
+// beyond the reference: a diagonal preconditioner M^-1 (Jacobi, or the
+// learned scaling) and the flexible (Polak-Ribiere) beta.
+//
+// All scalars live in device memory.  Reductions are two-stage: a fixed grid
+// of kRedBlocks blocks writes partial sums, and every block of the consuming
+// kernel re-sums the same partials in the same order, so each block derives
+// bit-identical alpha / beta without a grid barrier.  Iteration k uses the
+// ping-pong slot k & 1 for r.z and the done flag, so no kernel reads a slot
+// that a concurrent block of the same kernel writes.  Once the done flag is
+// set, every later kernel of the solve returns at once; the host polls the
+// flag every few iterations.
+#include <hip/hip_runtime.h>
+
+#include "kernels.hpp"
+
+namespace v2g {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+enum : int { S_RZ0 = CG_RZ0, S_DONE0 = CG_DONE0, S_DONE1 = CG_DONE1, S_NRM = CG_NRM,
+             S_ITER = CG_ITER, S_ALPHA = CG_ALPHA, S_BETA = CG_BETA, S_PAP = CG_PAP,
+             S_TOL = CG_TOL, S_MAXIT = CG_MAXIT };
+// partial-sum arrays inside `partial` (kRedBlocks each)
+enum : int { P_PAP = 0, P_RZ = 1, P_RR = 2, P_ROZ = 3, P_COUNT = 4 };
+
+__device__ __forceinline__ double block_sum(double v, double *sh)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < kThreads / 64; i++) s += sh[i];
+    return s;  // valid in thread 0
+}
+
+// every block sums the kRedBlocks partials in the same fixed order
+__device__ __forceinline__ double all_sum(const double *__restrict__ part, double *sh)
+{
+    double v = 0.0;
+    for (int i = threadIdx.x; i < kRedBlocks; i += kThreads) v += part[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int i = 0; i < kThreads / 64; i++) s += sh[i];
+    __syncthreads();
+    return s;  // valid in every thread
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_init(int64_t n, const double *__restrict__ b, const double *__restrict__ Ap,
+       double *__restrict__ r, double *__restrict__ p, const double *__restrict__ minv,
+       double *__restrict__ z, double *__restrict__ partial)
+{
+    __shared__ double sh[kThreads / 64];
+    double rz = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kThreads) {
+        const double ri = b[i] - Ap[i];
+        r[i] = ri;
+        const double zi = minv ? minv[i] * ri : ri;
+        if (z) z[i] = zi;
+        p[i] = zi;
+        rz += ri * zi;
+    }
+    const double s = block_sum(rz, sh);
+    if (threadIdx.x == 0) partial[P_RZ * kRedBlocks + blockIdx.x] = s;
+}
+
+__global__ void k_init_fin(double *__restrict__ partial, double *__restrict__ scal, double tol,
+                           int maxiter)
+{
+    __shared__ double sh[kThreads / 64];
+    const double rz = all_sum(partial + P_RZ * kRedBlocks, sh);
+    if (threadIdx.x == 0) {
+        scal[S_RZ0] = rz;
+        scal[S_DONE0] = (maxiter <= 0) ? 1.0 : 0.0;
+        scal[S_DONE1] = 0.0;
+        scal[S_ITER] = 0.0;
+        scal[S_TOL] = tol;
+        scal[S_MAXIT] = (double)maxiter;
+        scal[S_NRM] = 0.0;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_dot(int64_t n, const double *__restrict__ a, const double *__restrict__ b,
+      double *__restrict__ partial, const double *__restrict__ scal, int parity)
+{
+    __shared__ double sh[kThreads / 64];
+    if (scal[S_DONE0 + parity] != 0.0) return;
+    double s = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kThreads)
+        s += a[i] * b[i];
+    const double t = block_sum(s, sh);
+    if (threadIdx.x == 0) partial[P_PAP * kRedBlocks + blockIdx.x] = t;
+}
+
+// alpha = rz / pap; y += alpha p; r -= alpha Ap; z = M^-1 r; partial r.z, r.r, r_old.z
+__global__ void __launch_bounds__(kThreads)
+k_step(int64_t n, double *__restrict__ y, double *__restrict__ r, const double *__restrict__ p,
+       const double *__restrict__ Ap, const double *__restrict__ minv, double *__restrict__ z,
+       double *__restrict__ partial, double *__restrict__ scal, int parity)
+{
+    __shared__ double sh[kThreads / 64];
+    if (scal[S_DONE0 + parity] != 0.0) return;
+    const double pap = all_sum(partial + P_PAP * kRedBlocks, sh);
+    const double alpha = scal[S_RZ0 + parity] / pap;
+    double rz = 0.0, rr = 0.0, roz = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kThreads) {
+        y[i] += alpha * p[i];
+        const double ro = r[i];
+        const double rn = ro - alpha * Ap[i];
+        r[i] = rn;
+        const double zn = minv ? minv[i] * rn : rn;
+        if (z) z[i] = zn;
+        rz += rn * zn;
+        rr += rn * rn;
+        roz += ro * zn;
+    }
+    double t = block_sum(rz, sh);
+    if (threadIdx.x == 0) partial[P_RZ * kRedBlocks + blockIdx.x] = t;
+    t = block_sum(rr, sh);
+    if (threadIdx.x == 0) partial[P_RR * kRedBlocks + blockIdx.x] = t;
+    t = block_sum(roz, sh);
+    if (threadIdx.x == 0) partial[P_ROZ * kRedBlocks + blockIdx.x] = t;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scal[S_ALPHA] = alpha;
+        scal[S_PAP] = pap;
+    }
+}
+
+// convergence test and p = z + beta p
+__global__ void __launch_bounds__(kThreads)
+k_finish(int64_t n, double *__restrict__ p, const double *__restrict__ r,
+         const double *__restrict__ z, double *__restrict__ partial, double *__restrict__ scal,
+         int parity, int flexible, int host_iter)
+{
+    __shared__ double sh[kThreads / 64];
+    if (scal[S_DONE0 + parity] != 0.0) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) scal[S_DONE0 + (parity ^ 1)] = 1.0;
+        return;
+    }
+    const double rz = all_sum(partial + P_RZ * kRedBlocks, sh);
+    const double rr = all_sum(partial + P_RR * kRedBlocks, sh);
+    const double roz = flexible ? all_sum(partial + P_ROZ * kRedBlocks, sh) : 0.0;
+    const double nrm = sqrt(rr);
+    const double rzold = scal[S_RZ0 + parity];
+    const bool conv = nrm < scal[S_TOL];
+    const int iter_next = host_iter + 1;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        scal[S_NRM] = nrm;
+        if (conv) {
+            scal[S_DONE0 + (parity ^ 1)] = 1.0;
+            scal[S_ITER] = (double)host_iter;
+        } else {
+            scal[S_ITER] = (double)iter_next;
+            scal[S_DONE0 + (parity ^ 1)] = (iter_next >= (int)scal[S_MAXIT]) ? 1.0 : 0.0;
+            scal[S_RZ0 + (parity ^ 1)] = rz;
+        }
+    }
+    if (conv) return;
+    const double beta = (rz - (flexible ? roz : 0.0)) / rzold;
+    if (blockIdx.x == 0 && threadIdx.x == 0) scal[S_BETA] = beta;
+    const double *zz = z ? z : r;
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kThreads)
+        p[i] = zz[i] + beta * p[i];
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_axpby(int64_t n, double a, const double *x, double b, const double *y, double *out)
+{
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kThreads)
+        out[i] = a * x[i] + (y ? b * y[i] : 0.0);
+}
+
+__global__ void __launch_bounds__(kThreads)
+k_dotp(int64_t n, const double *__restrict__ a, const double *__restrict__ b,
+       double *__restrict__ partial)
+{
+    __shared__ double sh[kThreads / 64];
+    double s = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kThreads)
+        s += a[i] * b[i];
+    const double t = block_sum(s, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+__global__ void k_reduce(const double *__restrict__ partial, double *__restrict__ out)
+{
+    __shared__ double sh[kThreads / 64];
+    const double s = all_sum(partial, sh);
+    if (threadIdx.x == 0) *out = s;
+}
+
+}  // namespace
+
+// public wrappers (declared in kernels.hpp); `scal` must hold 32 doubles and
+// `partial` P_COUNT * kRedBlocks doubles
+void cg_init(int64_t n, const double *b, const double *Ap, double *r, double *p, const double *minv,
+             double *z, double *partial, double *scal, double tol, int maxiter, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_init, dim3(kRedBlocks), dim3(kThreads), 0, st, n, b, Ap, r, p, minv, z,
+                       partial);
+    hipLaunchKernelGGL(k_init_fin, dim3(1), dim3(kThreads), 0, st, partial, scal, tol, maxiter);
+}
+
+void cg_dot(int64_t n, const double *a, const double *b, double *partial, double *scal, int parity,
+            hipStream_t st)
+{
+    hipLaunchKernelGGL(k_dot, dim3(kRedBlocks), dim3(kThreads), 0, st, n, a, b, partial, scal,
+                       parity);
+}
+
+void cg_step(int64_t n, double *y, double *r, double *p, const double *Ap, const double *minv,
+             double *z, double *rprev, double *partial, double *scal, int parity, hipStream_t st)
+{
+    (void)rprev;
+    hipLaunchKernelGGL(k_step, dim3(kRedBlocks), dim3(kThreads), 0, st, n, y, r, p, Ap, minv, z,
+                       partial, scal, parity);
+}
+
+void cg_finish_iter(int64_t n, double *r, double *p, const double *z, double *partial, double *scal,
+                    int parity, int flexible, int host_iter, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_finish, dim3(kRedBlocks), dim3(kThreads), 0, st, n, p, r, z, partial, scal,
+                       parity, flexible, host_iter);
+}
+
+void vec_axpby(int64_t n, double a, const double *x, double b, const double *y, double *out,
+               hipStream_t st)
+{
+    hipLaunchKernelGGL(k_axpby, dim3(kRedBlocks), dim3(kThreads), 0, st, n, a, x, b, y, out);
+}
+
+void vec_dot_partial(int64_t n, const double *a, const double *b, double *partial, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_dotp, dim3(kRedBlocks), dim3(kThreads), 0, st, n, a, b, partial);
+}
+
+void vec_reduce(const double *partial, double *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(kThreads), 0, st, partial, out);
+}
+
+}  // namespace v2g

@@ -1,0 +1,45 @@
+// eig.hpp -- launch wrappers of the FP64 eigen split (eig.hip) and the
+// batched FP64 GEMM (gemm.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace v2g {
+
+struct EigBlock {
+    int n;          // block dimension
+    int N;          // padded dimension (multiple of 64) for the block-Jacobi path
+    int64_t xoff;   // offset of the block in x / z / A^T y
+    int64_t woff;   // offset in the N x N workspaces
+    int64_t qoff;   // offset in the pair-rotation buffer (N/64 pairs of 64 x 64)
+    int64_t poff;   // offset in the pair-active flags
+};
+
+struct GemmDesc {
+    const double *A;
+    const double *B;
+    double *C;
+    int M, N, K;
+    int lda, ldb, ldc;
+    int ta, tb;     // 1: use the transpose of A / B
+};
+
+void eig_small(const EigBlock *blocks, int nblocks, const double *aty, const double *c, double *x,
+               double *z, double mu, hipStream_t st);
+void eig_build_m(const EigBlock *blocks, int nblocks, const double *aty, const double *c,
+                 const double *x, double mu, double *Mw, double *Aw, double *Vw, int cold,
+                 hipStream_t st);
+void eig_tol(const EigBlock *blocks, int nblocks, const double *Mw, double *tol, hipStream_t st);
+void eig_round(const EigBlock *blocks, int nblocks, int maxpairs, int64_t maxtiles, int round,
+               double *Aw, double *Vw, double *Qbuf, int *active, const double *tol, int *counter,
+               hipStream_t st);
+void eig_scale(const EigBlock *blocks, int nblocks, const double *Aw, const double *Vw, double *Tw,
+               double mu, hipStream_t st);
+void eig_z(const EigBlock *blocks, int nblocks, const double *x, const double *Mw, double *z,
+           double mu, hipStream_t st);
+
+// C = op(A) op(B) for every descriptor (beta = 0); grid sized for the largest
+void gemm_batched(const GemmDesc *descs, int ndesc, int max_m, int max_n, hipStream_t st);
+
+}  // namespace v2g

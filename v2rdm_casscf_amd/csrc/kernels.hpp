@@ -1,0 +1,61 @@
+// kernels.hpp -- launch wrappers of the HIP kernels (maps, CG, eigen split,
+// learned preconditioner).  Every wrapper enqueues on the given stream and
+// returns without synchronizing.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "devlayout.hpp"
+
+namespace v2g {
+
+struct MapPlan {
+    const DevLayout *dlay = nullptr;
+    const Segment *rowsegs = nullptr;
+    const int64_t *row_tile_start = nullptr;
+    int nrowsegs = 0;
+    int64_t row_tiles = 0;
+    const Segment *xsegs = nullptr;
+    const int64_t *x_tile_start = nullptr;
+    int nxsegs = 0;
+    int64_t x_tiles = 0;
+    const Segment *t2segs = nullptr;
+    const int64_t *t2_tile_start = nullptr;
+    int nt2segs = 0;
+    int64_t t2_tiles = 0;
+};
+
+int64_t tiles_for(const Segment &s);
+void launch_au(const MapPlan &p, const double *x, double *Ax, hipStream_t st,
+               const double *done = nullptr);
+void launch_atu(const MapPlan &p, const double *y, double *ATy, hipStream_t st,
+                const double *done = nullptr);
+void launch_row_norm2(const MapPlan &p, double *d, hipStream_t st);
+
+// ---- CG (cg.hip) ----
+// device scalar slots of one CG solve; r.z and the done flag are ping-pong
+// pairs indexed by the iteration parity
+enum CgSlot : int {
+    CG_RZ0 = 16, CG_RZ1 = 17, CG_DONE0 = 18, CG_DONE1 = 19, CG_NRM = 20, CG_ITER = 21,
+    CG_ALPHA = 22, CG_BETA = 23, CG_PAP = 24, CG_TOL = 25, CG_MAXIT = 26, CG_NSLOTS = 32
+};
+constexpr int kRedBlocks = 512;     // partial-sum slots per reduction
+constexpr int kPartialArrays = 4;   // partial buffer holds 4 * kRedBlocks doubles
+
+void cg_init(int64_t n, const double *b, const double *Ap, double *r, double *p, const double *minv,
+             double *z, double *partial, double *scal, double tol, int maxiter, hipStream_t st);
+void cg_dot(int64_t n, const double *a, const double *b, double *partial, double *scal, int parity,
+            hipStream_t st);
+void cg_step(int64_t n, double *y, double *r, double *p, const double *Ap, const double *minv,
+             double *z, double *rprev, double *partial, double *scal, int parity, hipStream_t st);
+void cg_finish_iter(int64_t n, double *r, double *p, const double *z, double *partial, double *scal,
+                    int parity, int flexible, int host_iter, hipStream_t st);
+
+// ---- vector helpers (cg.hip) ----
+void vec_axpby(int64_t n, double a, const double *x, double b, const double *y, double *out,
+               hipStream_t st);
+void vec_dot_partial(int64_t n, const double *a, const double *b, double *partial, hipStream_t st);
+void vec_reduce(const double *partial, double *out, hipStream_t st);
+
+}  // namespace v2g

@@ -1,0 +1,328 @@
+// layout.cc -- see layout.hpp.
+#include "layout.hpp"
+
+#include <stdexcept>
+
+namespace v2g {
+
+Layout::Layout(const Options &o) : opt(o)
+{
+    if (o.nirrep != 1 && o.nirrep != 2 && o.nirrep != 4 && o.nirrep != 8)
+        throw std::invalid_argument("nirrep must be 1, 2, 4 or 8");
+    nirrep = o.nirrep;
+    na = o.nalpha;
+    nb = o.nbeta;
+    singlet = (o.nalpha == o.nbeta);
+    for (int h = 0; h < XB_COUNT; h++)
+        for (int i = 0; i < kMaxIrrep; i++) xoff[h][i] = -1;
+    for (int f = 0; f < RF_COUNT; f++)
+        for (int s = 0; s < 6; s++)
+            for (int h = 0; h < kMaxIrrep; h++) rowoff[f][s][h] = -1;
+    build_tables();
+    build_offsets();
+    build_rows();
+    build_xsegs();
+}
+
+void Layout::build_tables()
+{
+    amo = 0;
+    for (int h = 0; h < nirrep; h++) {
+        poff[h] = amo;
+        for (int i = 0; i < opt.amopi[h]; i++) sym.push_back(h);
+        amo += opt.amopi[h];
+    }
+    if (amo > 255) throw std::invalid_argument("at most 255 active orbitals");
+    gab.assign(amo * amo, -1);
+    gaa.assign(amo * amo, -1);
+    // Geminal n of irrep h: outer orbital loop i, inner j, pair (first=j,
+    // second=i); aa geminals keep first > second (basis.cc:439-548).
+    for (int h = 0; h < nirrep; h++) {
+        bab_off[h] = (int64_t)bab.size();
+        baa_off[h] = (int64_t)baa.size();
+        int nab = 0, naa = 0;
+        for (int i = 0; i < amo; i++)
+            for (int j = 0; j < amo; j++) {
+                if ((sym[i] ^ sym[j]) != h) continue;
+                const int first = j, second = i;
+                gab[first * amo + second] = nab++;
+                bab.push_back(first | (second << 16));
+                if (first <= second) continue;
+                gaa[first * amo + second] = naa;
+                gaa[second * amo + first] = naa;
+                naa++;
+                baa.push_back(first | (second << 16));
+            }
+        gems_ab[h] = nab;
+        gems_aa[h] = naa;
+    }
+    if (opt.t2) {
+        if (amo > 255) throw std::invalid_argument("T2 needs amo < 256");
+        // triplets: i outer, j, k inner; aab keeps i < j (basis.cc:655-757)
+        for (int h = 0; h < nirrep; h++) {
+            baab_off[h] = (int64_t)baab.size();
+            baba_off[h] = (int64_t)baba.size();
+            int caab = 0, caba = 0;
+            for (int i = 0; i < amo; i++)
+                for (int j = 0; j < amo; j++)
+                    for (int k = 0; k < amo; k++) {
+                        if ((sym[i] ^ sym[j] ^ sym[k]) != h) continue;
+                        const int packed = i | (j << 8) | (k << 16);
+                        baba.push_back(packed);
+                        caba++;
+                        if (i >= j) continue;
+                        baab.push_back(packed);
+                        caab++;
+                    }
+            trip_aab[h] = caab;
+            trip_aba[h] = caba;
+        }
+    }
+    if (bab.empty()) bab.push_back(0);
+    if (baa.empty()) baa.push_back(0);
+    if (baab.empty()) baab.push_back(0);
+    if (baba.empty()) baba.push_back(0);
+}
+
+void Layout::build_offsets()
+{
+    int64_t o = 0;
+    auto blocks = [&](int w, auto dimf) {
+        for (int h = 0; h < nirrep; h++) {
+            int64_t d = dimf(h);
+            xoff[w][h] = o;
+            o += d * d;
+            block_dims.push_back((int)d);
+            block_offs.push_back(xoff[w][h]);
+        }
+    };
+    auto g = [&](int h) { return (int64_t)gems_ab[h]; };
+    auto ga = [&](int h) { return (int64_t)gems_aa[h]; };
+    auto n = [&](int h) { return (int64_t)opt.amopi[h]; };
+    blocks(XB_D2AB, g);
+    blocks(XB_D2AA, ga);
+    blocks(XB_D2BB, ga);
+    if (opt.spin) {
+        if (singlet) blocks(XB_D200, g);
+        else blocks(XB_D200, [&](int h) { return 2 * g(h); });
+    }
+    blocks(XB_D1A, n);
+    blocks(XB_D1B, n);
+    blocks(XB_Q1A, n);
+    blocks(XB_Q1B, n);
+    if (opt.q2) {
+        blocks(XB_Q2AB, g);
+        blocks(XB_Q2AA, ga);
+        blocks(XB_Q2BB, ga);
+    }
+    if (opt.g2) {
+        blocks(XB_G2AB, g);
+        blocks(XB_G2BA, g);
+        blocks(XB_G2AA, [&](int h) { return 2 * g(h); });
+    }
+    if (opt.t2) {
+        auto T = [&](int h) { return (int64_t)trip_aab[h] + trip_aba[h]; };
+        auto t = [&](int h) { return (int64_t)trip_aab[h]; };
+        blocks(XB_T2AAA, T);
+        blocks(XB_T2BBB, T);
+        blocks(XB_T2AAB, t);
+        blocks(XB_T2BBA, t);
+    }
+    dimx = o;
+}
+
+void Layout::build_rows()
+{
+    int64_t r = 0;
+    auto seg = [&](int fam, int sub, int h, int dim, int64_t count, int wave) {
+        rowoff[fam][sub][h] = r;
+        if (count > 0) {
+            Segment s{};
+            s.kind = fam;
+            s.sub = sub;
+            s.h = h;
+            s.dim = dim;
+            s.start = r;
+            s.count = count;
+            s.wave = wave;
+            rowsegs.push_back(s);
+        }
+        r += count;
+    };
+    // D2 family, d2.cc:296-547
+    if (opt.sz) {
+        for (int s = 0; s < 3; s++) seg(RF_TRACE, s, 0, 0, 1, 1);
+    } else {
+        seg(RF_TRACE, 3, 0, 0, 1, 1);
+    }
+    for (int s = 0; s < 2; s++)
+        for (int h = 0; h < nirrep; h++)
+            seg(RF_D1Q1, s, h, opt.amopi[h], (int64_t)opt.amopi[h] * opt.amopi[h], 0);
+    const int nk = opt.sz ? 4 : 2;
+    for (int kk = 0; kk < nk; kk++) {
+        const int sub = opt.sz ? kk : 4 + kk;
+        for (int h = 0; h < nirrep; h++)
+            seg(RF_CONTR, sub, h, opt.amopi[h], (int64_t)opt.amopi[h] * opt.amopi[h], 1);
+    }
+    // Spin family, spin.cc:274-493
+    if (opt.spin) {
+        seg(RF_S2, 0, 0, 0, 1, 1);
+        if (singlet) {
+            for (int h = 0; h < nirrep; h++)
+                seg(RF_SPIN_D1, 0, h, opt.amopi[h], (int64_t)opt.amopi[h] * opt.amopi[h], 0);
+            for (int h = 0; h < nirrep; h++)
+                seg(RF_SPIN_D2, 0, h, gems_aa[h], (int64_t)gems_aa[h] * gems_aa[h], 0);
+            for (int s = 0; s < 2; s++)
+                for (int h = 0; h < nirrep; h++)
+                    seg(RF_SPIN_ANTI, s, h, gems_aa[h], (int64_t)gems_aa[h] * gems_aa[h], 0);
+            for (int h = 0; h < nirrep; h++)
+                seg(RF_SPIN_D200, 0, h, gems_ab[h], (int64_t)gems_ab[h] * gems_ab[h], 0);
+        } else {
+            for (int h = 0; h < nirrep; h++)
+                seg(RF_SPIN_D200_4, 0, h, gems_ab[h], 4LL * gems_ab[h] * gems_ab[h], 0);
+        }
+        if (opt.g2) {
+            seg(RF_MAXSPIN, 0, 0, gems_ab[0], gems_ab[0], 1);
+            seg(RF_MAXSPIN, 1, 0, gems_ab[0], gems_ab[0], 1);
+        }
+    }
+    // Q2, q2.cc:801-950
+    if (opt.q2) {
+        for (int h = 0; h < nirrep; h++)
+            seg(RF_Q2AB, 0, h, gems_ab[h], (int64_t)gems_ab[h] * gems_ab[h], 0);
+        for (int s = 0; s < 2; s++)
+            for (int h = 0; h < nirrep; h++)
+                seg(RF_Q2AA, s, h, gems_aa[h], (int64_t)gems_aa[h] * gems_aa[h], 0);
+    }
+    // G2, g2.cc:1061-1260
+    if (opt.g2) {
+        for (int s = 0; s < 2; s++)
+            for (int h = 0; h < nirrep; h++)
+                seg(RF_G2AB, s, h, gems_ab[h], (int64_t)gems_ab[h] * gems_ab[h], 0);
+        for (int h = 0; h < nirrep; h++)
+            seg(RF_G2AA, 0, h, gems_ab[h], 4LL * gems_ab[h] * gems_ab[h], 0);
+    }
+    // T2 slow maps: aab, bba, aaa(+abb), bbb(+baa); t2.cc:1251-1861
+    if (opt.t2) {
+        for (int s = 0; s < 2; s++)
+            for (int h = 0; h < nirrep; h++)
+                seg(RF_T2AAB, s, h, trip_aab[h], (int64_t)trip_aab[h] * trip_aab[h], 0);
+        for (int s = 0; s < 2; s++)
+            for (int h = 0; h < nirrep; h++) {
+                int64_t T = (int64_t)trip_aab[h] + trip_aba[h];
+                seg(RF_T2BIG, s, h, (int)T, T * T, 0);
+            }
+    }
+    ncon = r;
+}
+
+void Layout::build_xsegs()
+{
+    auto add = [&](std::vector<Segment> &v, int kind, int sub, int h, int dim, int64_t start,
+                   int64_t count, int64_t aux, double coef, int wave) {
+        if (count <= 0) return;
+        Segment s{};
+        s.kind = kind;
+        s.sub = sub;
+        s.h = h;
+        s.dim = dim;
+        s.start = start;
+        s.count = count;
+        s.aux = aux;
+        s.coef = coef;
+        s.wave = wave;
+        v.push_back(s);
+    };
+    for (int h = 0; h < nirrep; h++)
+        add(xsegs, XS_D2AB, 0, h, gems_ab[h], xoff[XB_D2AB][h],
+            (int64_t)gems_ab[h] * gems_ab[h], 0, 0.0, 0);
+    for (int s = 0; s < 2; s++)
+        for (int h = 0; h < nirrep; h++)
+            add(xsegs, XS_D2AA, s, h, gems_aa[h], xoff[s ? XB_D2BB : XB_D2AA][h],
+                (int64_t)gems_aa[h] * gems_aa[h], 0, 0.0, 0);
+    if (opt.spin) {
+        for (int h = 0; h < nirrep; h++) {
+            int64_t g = gems_ab[h];
+            if (singlet)
+                add(xsegs, XS_SELF, 0, h, (int)g, xoff[XB_D200][h], g * g,
+                    rowoff[RF_SPIN_D200][0][h], 1.0, 0);
+            else
+                add(xsegs, XS_SELF, 0, h, (int)(2 * g), xoff[XB_D200][h], 4 * g * g,
+                    rowoff[RF_SPIN_D200_4][0][h], 1.0, 0);
+        }
+    }
+    for (int s = 0; s < 2; s++)
+        for (int h = 0; h < nirrep; h++) {
+            int64_t n = opt.amopi[h];
+            add(xsegs, XS_D1, s, h, (int)n, xoff[s ? XB_D1B : XB_D1A][h], n * n, 0, 0.0, 1);
+        }
+    for (int s = 0; s < 2; s++)
+        for (int h = 0; h < nirrep; h++) {
+            int64_t n = opt.amopi[h];
+            add(xsegs, XS_SELF, 0, h, (int)n, xoff[s ? XB_Q1B : XB_Q1A][h], n * n,
+                rowoff[RF_D1Q1][s][h], 1.0, 0);
+        }
+    if (opt.q2) {
+        for (int h = 0; h < nirrep; h++) {
+            int64_t g = gems_ab[h];
+            add(xsegs, XS_SELF, 0, h, (int)g, xoff[XB_Q2AB][h], g * g, rowoff[RF_Q2AB][0][h],
+                -1.0, 0);
+        }
+        for (int s = 0; s < 2; s++)
+            for (int h = 0; h < nirrep; h++) {
+                int64_t g = gems_aa[h];
+                add(xsegs, XS_SELF, 0, h, (int)g, xoff[s ? XB_Q2BB : XB_Q2AA][h], g * g,
+                    rowoff[RF_Q2AA][s][h], -1.0, 0);
+            }
+    }
+    if (opt.g2) {
+        for (int s = 0; s < 2; s++)
+            for (int h = 0; h < nirrep; h++) {
+                int64_t g = gems_ab[h];
+                // sub 1: G2ba of irrep 0 also collects the maximal-spin rows
+                int sub = (s == 1 && h == 0 && opt.spin) ? 1 : 0;
+                add(xsegs, XS_SELF, sub, h, (int)g, xoff[s ? XB_G2BA : XB_G2AB][h], g * g,
+                    rowoff[RF_G2AB][s][h], -1.0, 0);
+            }
+        for (int h = 0; h < nirrep; h++) {
+            int64_t g = gems_ab[h];
+            add(xsegs, XS_SELF, 0, h, (int)(2 * g), xoff[XB_G2AA][h], 4 * g * g,
+                rowoff[RF_G2AA][0][h], -1.0, 0);
+        }
+    }
+    if (opt.t2) {
+        // the T2 pass walks the T2 rows: self term into the T2 block plus the
+        // delta-selected D2 / D1 terms scattered atomically
+        for (int s = 0; s < 2; s++)
+            for (int h = 0; h < nirrep; h++) {
+                int64_t t = trip_aab[h];
+                add(xsegs_t2, RF_T2AAB, s, h, (int)t, xoff[s ? XB_T2BBA : XB_T2AAB][h], t * t,
+                    rowoff[RF_T2AAB][s][h], -1.0, 0);
+            }
+        for (int s = 0; s < 2; s++)
+            for (int h = 0; h < nirrep; h++) {
+                int64_t T = (int64_t)trip_aab[h] + trip_aba[h];
+                add(xsegs_t2, RF_T2BIG, s, h, (int)T, xoff[s ? XB_T2BBB : XB_T2AAA][h], T * T,
+                    rowoff[RF_T2BIG][s][h], -1.0, 0);
+            }
+    }
+}
+
+std::vector<std::pair<std::string, std::pair<int64_t, int64_t>>> Layout::families() const
+{
+    static const char *names[RF_COUNT] = {
+        "trace", "d1q1", "contract", "s2", "spin_d1", "spin_d2", "spin_anti", "spin_d200",
+        "spin_d200_4", "maxspin", "q2ab", "q2aa", "g2ab", "g2aa", "t2aab", "t2big"};
+    std::vector<std::pair<std::string, std::pair<int64_t, int64_t>>> out;
+    for (const auto &s : rowsegs) {
+        std::string nm = names[s.kind];
+        if (!out.empty() && out.back().first == nm &&
+            out.back().second.first + out.back().second.second == s.start)
+            out.back().second.second += s.count;
+        else
+            out.push_back({nm, {s.start, s.count}});
+    }
+    return out;
+}
+
+}  // namespace v2g

@@ -1,0 +1,214 @@
+// maps.hip -- the BPSDP constraint maps A.x and A^T.y on CDNA4.
+//
+// A.x (bpsdp_Au, v2rdm_solver.cc:3020-3065): one launch.  Every constraint
+// row is a gather over a handful of primal entries, computed by one thread;
+// the rows whose sums run over all orbitals (traces, S^2, D2->D1
+// contractions, maximal-spin rows) get one 64-lane wavefront each.
+//
+// A^T.y (bpsdp_ATu, v2rdm_solver.cc:3114-3158): also a gather, one thread per
+// primal entry, so the result is deterministic and needs no atomics:
+//   * "self" blocks (D200, Q1, Q2, G2): each entry is touched by exactly one
+//     row with coefficient +-1, laid out identically -> a coalesced copy;
+//   * D2ab / D2aa / D2bb entries invert each family's index rule in closed
+//     form (the particle-hole G2 maps are permutations per irrep);
+//   * D1 entries sum O(amo) rows -> one wavefront per entry.
+// With T2 on, a second launch walks the T2 rows: the T2 block itself is a
+// self block, and the rare delta-selected D2 / D1 terms of a T2 row are added
+// with hardware FP64 atomics.
+//
+// Row formulas follow d2.cc:296-547, spin.cc:274-493, q2.cc:801-950,
+// g2.cc:1061-1260 and t2.cc:1251-1861.
+#include <hip/hip_runtime.h>
+
+#include "devlayout.hpp"
+#include "kernels.hpp"
+#include "rowterms.hpp"
+
+namespace v2g {
+
+namespace {
+
+using namespace rows;
+
+constexpr int kBlock = 256;
+constexpr int kWavesPerBlock = kBlock / 64;
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+struct Gather {
+    const double *__restrict__ x;
+    double acc;
+    __device__ __forceinline__ void operator()(int64_t col, double coef) { acc += coef * x[col]; }
+};
+
+struct Scatter {
+    double *__restrict__ out;
+    double y;
+    __device__ __forceinline__ void operator()(int64_t col, double coef)
+    {
+        unsafeAtomicAdd(out + col, coef * y);
+    }
+};
+
+__device__ __forceinline__ int find_segment(const int64_t *__restrict__ tile_start, int nseg,
+                                            int64_t b)
+{
+    int lo = 0, hi = nseg;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (tile_start[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_au(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
+     const int64_t *__restrict__ tile_start, int nseg, const double *__restrict__ x,
+     double *__restrict__ Ax, const double *__restrict__ done)
+{
+    if (done && *done != 0.0) return;
+    const DevLayout &L = *Lp;
+    const int si = find_segment(tile_start, nseg, blockIdx.x);
+    const Segment sg = segs[si];
+    const int64_t t = blockIdx.x - tile_start[si];
+    if (sg.wave) {
+        const int64_t item = t * kWavesPerBlock + (threadIdx.x >> 6);
+        if (item >= sg.count) return;
+        const int lane = threadIdx.x & 63;
+        Gather f{x, 0.0};
+        wave_terms(L, sg, item, lane, f);
+        const double v = wave_sum(f.acc);
+        if (lane == 0) Ax[sg.start + item] = v;
+    } else {
+        const int64_t e = t * kBlock + threadIdx.x;
+        if (e >= sg.count) return;
+        Gather f{x, 0.0};
+        elem_terms(L, sg, e, f);
+        Ax[sg.start + e] = f.acc;
+    }
+}
+
+// diag(A A^T): squared norm of every constraint row (Jacobi preconditioner,
+// features of the learned preconditioner).  Trace rows repeat columns across
+// lanes and are given in closed form.
+__global__ void __launch_bounds__(kBlock)
+k_row_norm2(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
+            const int64_t *__restrict__ tile_start, int nseg, double *__restrict__ d)
+{
+    const DevLayout &L = *Lp;
+    const int si = find_segment(tile_start, nseg, blockIdx.x);
+    const Segment sg = segs[si];
+    const int64_t t = blockIdx.x - tile_start[si];
+    if (sg.wave) {
+        const int64_t item = t * kWavesPerBlock + (threadIdx.x >> 6);
+        if (item >= sg.count) return;
+        const int lane = threadIdx.x & 63;
+        double v;
+        if (sg.kind == RF_TRACE) {
+            const double a = L.amo, npair = a * (a - 1.0) / 2.0;
+            // ab diagonal: a^2 entries of weight 1 (or 2); aa diagonal: each
+            // geminal appears for (i,j) and (j,i) -> weight 2
+            if (sg.sub == 0) v = a * a;
+            else if (sg.sub < 3) v = 4.0 * npair;
+            else v = 4.0 * a * a + 2.0 * 4.0 * npair;
+        } else {
+            SumSq f;
+            wave_terms(L, sg, item, lane, f);
+            v = wave_sum(f.acc);
+        }
+        if (lane == 0) d[sg.start + item] = v;
+    } else {
+        const int64_t e = t * kBlock + threadIdx.x;
+        if (e >= sg.count) return;
+        Collect f;
+        elem_terms(L, sg, e, f);
+        d[sg.start + e] = f.norm2();
+    }
+}
+
+__global__ void __launch_bounds__(kBlock)
+k_atu(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
+      const int64_t *__restrict__ tile_start, int nseg, const double *__restrict__ y,
+      double *__restrict__ ATy, const double *__restrict__ done)
+{
+    if (done && *done != 0.0) return;
+    const DevLayout &L = *Lp;
+    const int si = find_segment(tile_start, nseg, blockIdx.x);
+    const Segment sg = segs[si];
+    const int64_t t = blockIdx.x - tile_start[si];
+    if (sg.wave) {
+        const int64_t item = t * kWavesPerBlock + (threadIdx.x >> 6);
+        if (item >= sg.count) return;
+        const int lane = threadIdx.x & 63;
+        const double v = wave_sum(atu_d1_wave(L, sg, item, lane, y));
+        if (lane == 0) ATy[sg.start + item] = v;
+        return;
+    }
+    const int64_t e = t * kBlock + threadIdx.x;
+    if (e >= sg.count) return;
+    double v;
+    switch (sg.kind) {
+    case XS_D2AB: v = atu_d2ab(L, sg, e, y); break;
+    case XS_D2AA: v = atu_d2aa(L, sg, e, y); break;
+    default: v = atu_self(L, sg, e, y); break;
+    }
+    ATy[sg.start + e] = v;
+}
+
+// T2 rows: self entry written, delta-selected D2 / D1 entries accumulated
+__global__ void __launch_bounds__(kBlock)
+k_atu_t2(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
+         const int64_t *__restrict__ tile_start, int nseg, const double *__restrict__ y,
+         double *__restrict__ ATy, const double *__restrict__ done)
+{
+    if (done && *done != 0.0) return;
+    const DevLayout &L = *Lp;
+    const int si = find_segment(tile_start, nseg, blockIdx.x);
+    const Segment sg = segs[si];
+    const int64_t e = (blockIdx.x - tile_start[si]) * kBlock + threadIdx.x;
+    if (e >= sg.count) return;
+    const double yv = y[sg.aux + e];
+    ATy[sg.start + e] = -yv;
+    Scatter f{ATy, yv};
+    t2_terms(L, sg.kind, sg.sub, sg.h, e, f, false);
+}
+
+}  // namespace
+
+void launch_au(const MapPlan &p, const double *x, double *Ax, hipStream_t st, const double *done)
+{
+    if (p.row_tiles > 0)
+        hipLaunchKernelGGL(k_au, dim3((unsigned)p.row_tiles), dim3(kBlock), 0, st, p.dlay,
+                           p.rowsegs, p.row_tile_start, p.nrowsegs, x, Ax, done);
+}
+
+void launch_row_norm2(const MapPlan &p, double *d, hipStream_t st)
+{
+    if (p.row_tiles > 0)
+        hipLaunchKernelGGL(k_row_norm2, dim3((unsigned)p.row_tiles), dim3(kBlock), 0, st, p.dlay,
+                           p.rowsegs, p.row_tile_start, p.nrowsegs, d);
+}
+
+void launch_atu(const MapPlan &p, const double *y, double *ATy, hipStream_t st, const double *done)
+{
+    if (p.x_tiles > 0)
+        hipLaunchKernelGGL(k_atu, dim3((unsigned)p.x_tiles), dim3(kBlock), 0, st, p.dlay,
+                           p.xsegs, p.x_tile_start, p.nxsegs, y, ATy, done);
+    if (p.t2_tiles > 0)
+        hipLaunchKernelGGL(k_atu_t2, dim3((unsigned)p.t2_tiles), dim3(kBlock), 0, st, p.dlay,
+                           p.t2segs, p.t2_tile_start, p.nt2segs, y, ATy, done);
+}
+
+int64_t tiles_for(const Segment &s)
+{
+    const int64_t per = s.wave ? kWavesPerBlock : kBlock;
+    return (s.count + per - 1) / per;
+}
+
+}  // namespace v2g

@@ -1,0 +1,233 @@
+"""Host-side mirror of the reference's v2RDMSolver hot path over the v2g C ABI.
+
+Method names and argument meaning follow the reference (v2rdm_solver.h:85,
+366-417; cg_solver.h:49-53):
+
+  bpsdp_Au(A, u)     -> A = A(u)                  v2rdm_solver.cc:3020-3065
+  bpsdp_ATu(A, u)    -> A = A^T(u)                v2rdm_solver.cc:3114-3158
+  cg_Ax(N, A, u)     -> A = A(A^T u)              v2rdm_solver.cc:3207-3213
+  cg_solve(...)      -> CGSolver::solve           cg_solver.cc:125-177
+  Update_xz()        -> per-block eigen split     v2rdm_solver.cc:3216-3289
+  BuildConstraints() -> b                         v2rdm_solver.cc:2518-3017
+  GetIntegrals(h,eri)-> c (RepackIntegrals)       tei.cc:144-313
+  Guess()            -> RANDOM guess              v2rdm_solver.cc:2386-2490
+  compute_energy()   -> BPSDP outer loop          v2rdm_solver.cc:1685-1912
+
+Vectors are torch float64 tensors on the HIP device (torch is used only to
+hold device memory).  Errors raise V2GError with the library's message, the
+analogue of the reference's PsiException.
+"""
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check
+
+POSITIVITY = {"D": (0, 0, 0), "DQ": (1, 0, 0), "DG": (0, 1, 0), "DQG": (1, 1, 0),
+              "DQGT2": (1, 1, 1)}
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _np_ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class V2RDMSolver:
+    """One active space / constraint set on one GPU (`device`)."""
+
+    def __init__(self, amopi, nalpha, nbeta, multiplicity=None, positivity="DQG",
+                 constrain_spin=True, constrain_sz=True, device="cuda:0", options=None):
+        pos = positivity.upper()
+        if pos not in POSITIVITY:
+            raise ValueError(f"POSITIVITY {positivity} is not supported (D, DQ, DG, DQG, DQGT2)")
+        amopi = list(amopi)
+        if len(amopi) not in (1, 2, 4, 8):
+            raise ValueError("amopi must have 1, 2, 4 or 8 irreps")
+        self.lib = _lib.load()
+        self.device = torch.device(device)
+        d = _lib.LayoutDesc()
+        d.nirrep = len(amopi)
+        for h in range(8):
+            d.amopi[h] = amopi[h] if h < len(amopi) else 0
+        d.nalpha, d.nbeta = int(nalpha), int(nbeta)
+        d.multiplicity = int(multiplicity) if multiplicity else int(nalpha - nbeta + 1)
+        d.constrain_q2, d.constrain_g2, d.constrain_t2 = POSITIVITY[pos]
+        d.constrain_spin, d.constrain_sz = int(constrain_spin), int(constrain_sz)
+        self.desc = d
+        self.amopi, self.nalpha, self.nbeta = amopi, nalpha, nbeta
+        self.positivity = pos
+        with torch.cuda.device(self.device):
+            h = ctypes.c_void_p()
+            check(self.lib.v2g_create(ctypes.byref(d), ctypes.byref(h)))
+        self.h = h
+        dx, nc, nb = ctypes.c_longlong(), ctypes.c_longlong(), ctypes.c_int()
+        check(self.lib.v2g_dims(self.h, ctypes.byref(dx), ctypes.byref(nc), ctypes.byref(nb)))
+        self.dimx, self.nconstraints = dx.value, nc.value
+        dims = (ctypes.c_int * nb.value)()
+        check(self.lib.v2g_block_dims(self.h, dims))
+        self.dimensions = list(dims)
+        # reference option defaults (plugin.cc:97-124), E/R_CONVERGENCE as the
+        # configurations here use them
+        self.options = dict(TAU_PARAMETER=1.0, R_CONVERGENCE=1e-6, E_CONVERGENCE=1e-6,
+                            CG_CONVERGENCE=1e-9, MAXITER=10000, CG_MAXITER=10000,
+                            MU_UPDATE_FREQUENCY=500, PRECONDITIONER="NONE", FLEXIBLE_CG=False)
+        if options:
+            self.options.update({k.upper(): v for k, v in options.items()})
+        self.x = self.z = self.y = self.c = self.b = None
+        self.mu = 1.0
+        self._cb = None
+
+    def __del__(self):
+        try:
+            if getattr(self, "h", None):
+                self.lib.v2g_destroy(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+    # ---- allocation helpers --------------------------------------------
+    def _vec(self, n):
+        return torch.empty(n, dtype=torch.float64, device=self.device)
+
+    def _check_vec(self, t, n, name):
+        if not (t.is_cuda and t.dtype == torch.float64 and t.is_contiguous() and t.numel() == n):
+            raise ValueError(f"{name}: expected a contiguous float64 device tensor of {n} entries")
+
+    def _sync(self):
+        torch.cuda.synchronize(self.device)
+
+    # ---- maps ----------------------------------------------------------
+    def bpsdp_Au(self, A, u):
+        self._check_vec(u, self.dimx, "u")
+        self._check_vec(A, self.nconstraints, "A")
+        self._sync()
+        check(self.lib.v2g_Au(self.h, _ptr(u), _ptr(A)))
+        return A
+
+    def bpsdp_ATu(self, A, u):
+        self._check_vec(u, self.nconstraints, "u")
+        self._check_vec(A, self.dimx, "A")
+        self._sync()
+        check(self.lib.v2g_ATu(self.h, _ptr(u), _ptr(A)))
+        return A
+
+    def cg_Ax(self, N, A, u):
+        if N != self.nconstraints:
+            raise ValueError("dimension does not match dimension from initialization")
+        self._check_vec(u, N, "u")
+        self._check_vec(A, N, "A")
+        self._sync()
+        check(self.lib.v2g_cg_Ax(self.h, _ptr(u), _ptr(A)))
+        return A
+
+    def row_norm2(self):
+        d = self._vec(self.nconstraints)
+        self._sync()
+        check(self.lib.v2g_row_norm2(self.h, _ptr(d)))
+        return d
+
+    # ---- CG ------------------------------------------------------------
+    def cg_solve(self, rhs, y, tol, maxiter, precond=0, flexible=False):
+        """CGSolver::solve on A A^T; y updated in place; returns (iters, ||r||)."""
+        self._check_vec(rhs, self.nconstraints, "rhs")
+        self._check_vec(y, self.nconstraints, "y")
+        self._sync()
+        it = ctypes.c_int()
+        res = ctypes.c_double()
+        check(self.lib.v2g_cg_solve(self.h, _ptr(rhs), _ptr(y), float(tol), int(maxiter),
+                                    int(precond), int(bool(flexible)), ctypes.byref(it),
+                                    ctypes.byref(res)))
+        return it.value, res.value
+
+    def set_precond_diag(self, minv):
+        self._check_vec(minv, self.nconstraints, "minv")
+        self._sync()
+        check(self.lib.v2g_set_precond_diag(self.h, _ptr(minv)))
+
+    # ---- Update_xz ------------------------------------------------------
+    def update_xz(self, mu, y, c, x, z):
+        for t, n, nm in ((y, self.nconstraints, "y"), (c, self.dimx, "c"), (x, self.dimx, "x"),
+                         (z, self.dimx, "z")):
+            self._check_vec(t, n, nm)
+        self._sync()
+        check(self.lib.v2g_update_xz(self.h, float(mu), _ptr(y), _ptr(c), _ptr(x), _ptr(z)))
+        return x, z
+
+    def Update_xz(self):
+        return self.update_xz(self.mu, self.y, self.c, self.x, self.z)
+
+    def reset_eig(self):
+        check(self.lib.v2g_reset_eig(self.h))
+
+    # ---- path inputs (host) -------------------------------------------
+    def BuildConstraints(self):
+        b = np.empty(self.nconstraints)
+        check(self.lib.v2g_build_b(self.h, _np_ptr(b)))
+        self.b = torch.from_numpy(b).to(self.device)
+        return self.b
+
+    def GetIntegrals(self, h1, eri):
+        """c from active-space integrals (h1: amo x amo, eri: amo^4 chemists')."""
+        amo = sum(self.amopi)
+        h1 = np.ascontiguousarray(h1, dtype=np.float64).reshape(amo, amo)
+        eri = np.ascontiguousarray(eri, dtype=np.float64).reshape(amo, amo, amo, amo)
+        c = np.empty(self.dimx)
+        check(self.lib.v2g_build_c(self.h, _np_ptr(h1), _np_ptr(eri), _np_ptr(c)))
+        self.c = torch.from_numpy(c).to(self.device)
+        return self.c
+
+    def Guess(self):
+        x = np.empty(self.dimx)
+        z = np.empty(self.dimx)
+        y = np.empty(self.nconstraints)
+        check(self.lib.v2g_guess_random(self.h, _np_ptr(x), _np_ptr(z), _np_ptr(y)))
+        self.x = torch.from_numpy(x).to(self.device)
+        self.z = torch.from_numpy(z).to(self.device)
+        self.y = torch.from_numpy(y).to(self.device)
+        return self.x, self.z, self.y
+
+    # ---- outer loop ---------------------------------------------------
+    def set_iter_callback(self, fn):
+        """fn(oiter, stats_dict) -> bool (True stops); None clears."""
+        if fn is None:
+            self._cb = None
+            check(self.lib.v2g_set_iter_callback(self.h, _lib.ITER_CALLBACK(0), None))
+            return
+
+        def tramp(user, oiter, st):
+            s = st.contents
+            return int(bool(fn(oiter, {k: getattr(s, k) for k, _ in _lib.Stats._fields_})))
+
+        self._cb = _lib.ITER_CALLBACK(tramp)
+        check(self.lib.v2g_set_iter_callback(self.h, self._cb, None))
+
+    def compute_energy(self, h1=None, eri=None, enuc=0.0, verbose=False, max_seconds=0.0,
+                       guess=True):
+        """BPSDP loop; returns a stats dict with energies (enuc added)."""
+        if h1 is not None:
+            self.GetIntegrals(h1, eri)
+        if self.c is None:
+            raise ValueError("no integrals: pass h1/eri or call GetIntegrals first")
+        if self.b is None:
+            self.BuildConstraints()
+        if guess or self.x is None:
+            self.Guess()
+        o = self.options
+        pc = {"NONE": 0, "JACOBI": 1, "LEARNED": 2}[str(o["PRECONDITIONER"]).upper()]
+        p = _lib.Params(o["TAU_PARAMETER"], o["R_CONVERGENCE"], o["E_CONVERGENCE"],
+                        o["CG_CONVERGENCE"], int(o["MAXITER"]), int(o["CG_MAXITER"]),
+                        int(o["MU_UPDATE_FREQUENCY"]), pc, int(bool(o["FLEXIBLE_CG"])),
+                        int(bool(verbose)), float(max_seconds))
+        st = _lib.Stats()
+        self._sync()
+        check(self.lib.v2g_bpsdp(self.h, ctypes.byref(p), _ptr(self.c), _ptr(self.b),
+                                 _ptr(self.x), _ptr(self.z), _ptr(self.y), ctypes.byref(st)))
+        out = {k: getattr(st, k) for k, _ in _lib.Stats._fields_}
+        out["energy"] = out["energy_primal"] + enuc
+        self.mu = out["mu"]
+        return out
