@@ -1,0 +1,190 @@
+"""Benchmark: BPSDP outer iterations on one MI355X.
+
+Workload (default): linear H50 / STO-3G, spacing 1.8 bohr, D2h, (50e,50o)
+full active space, DQG + spin constraints, FP64 -- the configuration the
+headline metric names.  A step is one outer BPSDP iteration (rhs build, CG on
+A A^T, Update_xz eigen split, primal/dual errors), exactly as
+v2RDMSolver::compute_energy does it; `value` is outer iterations per second
+with every vector resident in HBM.
+
+Also reported on the same JSON line:
+  roofline      the A^T.y + A.x map kernels (the CG operator), timed with HIP
+                events on the engine's stream: algorithmic bytes per launch
+                8 (dimx + ncon) / mean launch time, against 8 TB/s HBM;
+  eig           the Update_xz eigen split timed the same way, with its
+                algorithmic FP64 flops sum (16/3) N^3 against 78.6 TFLOP/s;
+  cpu_baseline  the CPU oracle (OpenMP maps + MKL dsyev) on the host, on a
+                bounded sample (one CG operator application and one Update_xz
+                timed; an outer iteration projected with the GPU run's mean CG
+                iterations per outer iteration).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
+
+
+def _workload(name):
+    from v2rdm_casscf_amd.hamiltonians import hydrogen_chain, synthetic_cholesky
+    if name == "h50":
+        d = hydrogen_chain(50, 1.8)
+        return dict(name="H50 STO-3G r=1.8 bohr D2h (50e,50o) DQG+spin", amopi=d["amopi"],
+                    na=25, nb=25, h1=d["h1"], eri=d["eri"], enuc=d["enuc"])
+    if name.startswith("h"):
+        n = int(name[1:])
+        d = hydrogen_chain(n, 1.8)
+        return dict(name=f"H{n} STO-3G r=1.8 bohr D2h ({n}e,{n}o) DQG+spin", amopi=d["amopi"],
+                    na=n // 2, nb=n // 2, h1=d["h1"], eri=d["eri"], enuc=d["enuc"])
+    if name.startswith("syn"):
+        n = int(name[3:])
+        h1, eri = synthetic_cholesky(n, 0)
+        return dict(name=f"synthetic Cholesky n={n} C1 ({n}e,{n}o) DQG+spin", amopi=[n],
+                    na=n // 2, nb=n // 2, h1=h1, eri=eri, enuc=0.0)
+    raise ValueError(name)
+
+
+def _eig_flops(dims):
+    return sum((16.0 / 3.0) * float(n) ** 3 for n in dims)
+
+
+def cpu_baseline(w, gpu_cg_per_outer, seconds_budget=25.0):
+    """Time the oracle's CG operator and Update_xz on a bounded sample."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle_py as orc
+    L = orc.Layout(w["amopi"], w["na"], w["nb"], None, "DQG")
+    c = L.build_c(w["h1"], w["eri"])
+    x, z, y = L.guess_random()
+    t0 = time.time()
+    L.cg_Ax(y)
+    t_op = time.time() - t0
+    t0 = time.time()
+    L.update_xz(1.0, y, c, x)
+    t_xz = time.time() - t0
+    per_outer = t_xz + (gpu_cg_per_outer + 4.0) * t_op
+    return dict(value=1.0 / per_outer, unit="outer iter/s", cores=orc.num_threads(),
+                kind="port", lapack="MKL dsyev" if orc.lapack_backend() else "built-in Jacobi",
+                sample=(f"oracle on the same workload: one A(A^T y) = {t_op:.3f} s and one "
+                        f"Update_xz = {t_xz:.3f} s timed; outer iteration projected as "
+                        f"Update_xz + ({gpu_cg_per_outer:.1f} CG + 4 map) applications"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="h50")
+    ap.add_argument("--precond", default="jacobi", choices=["none", "jacobi", "learned"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--map-reps", type=int, default=20)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box in this setup has a single MI355X; the "
+              "multi-GPU sharding plan is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from v2rdm_casscf_amd import V2RDMSolver
+
+    dev = torch.device("cuda:0")
+    w = _workload(args.workload)
+    s = V2RDMSolver(w["amopi"], w["na"], w["nb"], 1, "DQG", device=dev,
+                    options=dict(PRECONDITIONER=args.precond.upper()))
+    stream = torch.cuda.Stream(device=dev)
+    s.lib.v2g_set_stream(s.h, __import__("ctypes").c_void_p(stream.cuda_stream))
+    s.GetIntegrals(w["h1"], w["eri"])
+    s.BuildConstraints()
+    s.Guess()
+    torch.cuda.synchronize()
+
+    # warmup outer iterations (includes the cold eigen sweeps of the guess)
+    s.options["MAXITER"] = args.warmup
+    st_w = s.compute_energy(guess=False)
+    # timed outer iterations: x, z, y continue from the warmup state
+    s.options["MAXITER"] = args.steps
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = s.compute_energy(guess=False)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    nsteps = st["oiter"]
+    ms_per_step = 1e3 * elapsed / nsteps
+
+    # per-kernel timing with HIP events on the engine's stream
+    ncon, dimx = s.nconstraints, s.dimx
+    yv = s.y.clone()
+    out = torch.empty(ncon, dtype=torch.float64, device=dev)
+    tmp = torch.empty(dimx, dtype=torch.float64, device=dev)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    with torch.cuda.stream(stream):
+        s.bpsdp_ATu(tmp, yv)
+        ev0.record(stream)
+        for _ in range(args.map_reps):
+            s.lib.v2g_ATu(s.h, __import__("ctypes").c_void_p(yv.data_ptr()),
+                          __import__("ctypes").c_void_p(tmp.data_ptr()))
+        ev1.record(stream)
+    ev1.synchronize()
+    t_atu = ev0.elapsed_time(ev1) / 1e3 / args.map_reps
+    with torch.cuda.stream(stream):
+        ev0.record(stream)
+        for _ in range(args.map_reps):
+            s.lib.v2g_Au(s.h, __import__("ctypes").c_void_p(tmp.data_ptr()),
+                         __import__("ctypes").c_void_p(out.data_ptr()))
+        ev1.record(stream)
+    ev1.synchronize()
+    t_au = ev0.elapsed_time(ev1) / 1e3 / args.map_reps
+    bytes_map = 8.0 * (dimx + ncon)
+    xs, zs = s.x.clone(), s.z.clone()
+    with torch.cuda.stream(stream):
+        ev0.record(stream)
+        s.lib.v2g_update_xz(s.h, __import__("ctypes").c_double(st["mu"]),
+                            __import__("ctypes").c_void_p(yv.data_ptr()),
+                            __import__("ctypes").c_void_p(s.c.data_ptr()),
+                            __import__("ctypes").c_void_p(xs.data_ptr()),
+                            __import__("ctypes").c_void_p(zs.data_ptr()))
+        ev1.record(stream)
+    ev1.synchronize()
+    t_xz = ev0.elapsed_time(ev1) / 1e3
+    eflops = _eig_flops([n for n in s.dimensions if n > 0])
+
+    cg_per_outer = st["cg_iters"] / max(1, nsteps)
+    roofline = dict(bound="hbm", kernel="A^T.y (k_atu) + A.x (k_au)",
+                    achieved=round(2 * bytes_map / (t_atu + t_au) / 1e9, 1), peak=HBM_PEAK_GBS,
+                    unit="GB/s", traffic=None, t_atu_ms=round(1e3 * t_atu, 4),
+                    t_au_ms=round(1e3 * t_au, 4), bytes_per_launch=bytes_map)
+    roofline["frac"] = round(roofline["achieved"] / HBM_PEAK_GBS, 4)
+    eig = dict(bound="mfma", achieved=round(eflops / t_xz / 1e12, 3), peak=FP64_PEAK_TFLOPS,
+               unit="TFLOP/s", t_update_xz_ms=round(1e3 * t_xz, 3), flops=eflops,
+               sweeps=st["eig_sweeps"])
+    eig["frac"] = round(eig["achieved"] / FP64_PEAK_TFLOPS, 4)
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(w, cg_per_outer)
+        except Exception as e:  # reported, not fatal
+            cpu = dict(error=str(e))
+    value = nsteps / elapsed
+    line = dict(metric="outer BPSDP iterations/s (time-to-1e-6Eh projected), H50 (50e,50o) DQG",
+                value=round(value, 4), unit="outer iter/s", n_gpus=1, steps=nsteps,
+                warmup=args.warmup, ms_per_step=round(ms_per_step, 3), higher_is_better=True,
+                scaling="weak", vs_baseline=None, dtype="f64", data="synthetic",
+                config=dict(workload=w["name"], dimx=dimx, ncon=ncon,
+                            precond=args.precond, cg_iters_per_outer=round(cg_per_outer, 2),
+                            energy=st["energy_primal"] + w["enuc"], ep=st["ep"], ed=st["ed"],
+                            mu=st["mu"], t_cg_s=st["t_cg"], t_xz_s=st["t_xz"],
+                            warmup_s=st_w["t_total"]),
+                roofline=roofline, eig=eig, cpu_baseline=cpu)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

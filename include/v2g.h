@@ -72,6 +72,25 @@ int v2g_update_xz(v2g_handle h, double mu, const double *y_dev, const double *c_
 /* forget the kept eigenvectors (next update_xz starts cold) */
 int v2g_reset_eig(v2g_handle h);
 
+/* ---- learned preconditioner (no reference counterpart) ----
+ * A per-row MLP (14 inputs: one-hot constraint family x11, log diag(A A^T),
+ * asinh(rhs/rms), asinh(y_prev/rms); tanh hidden layers 64, 64; one output g)
+ * gives M^-1_ii = exp(clamp(g_i, -3, 3)) / diag(A A^T)_ii.  Weights are FP32,
+ * packed W1[14][64], b1[64], W2[64][64], b2[64], w3[64], b3 (5185 floats). */
+int v2g_mlp_num_params(void);
+/* seeded init; the last layer starts at zero, i.e. exactly Jacobi */
+int v2g_mlp_init(v2g_handle h, unsigned long long seed);
+int v2g_mlp_get_weights(v2g_handle h, float *w_host);
+int v2g_mlp_set_weights(v2g_handle h, const float *w_host);
+/* loss = sum_b wt_b (out_b - tgt_b)^2 / nb and its gradient (FP64 out,
+ * device pointers; feat is nb x 14 FP32) -- the training kernels */
+int v2g_mlp_loss_grad(v2g_handle h, int nb, const float *feat_dev, const float *tgt_dev,
+                      const float *wt_dev, double *grad_dev, double *loss_host);
+/* one Adam step (lr) on the current weights with grad_dev */
+int v2g_mlp_adam_step(v2g_handle h, const double *grad_dev, double lr);
+/* inference: minv for every constraint row from rhs and y (device) */
+int v2g_mlp_minv(v2g_handle h, const double *rhs_dev, const double *y_dev, double *minv_dev);
+
 typedef struct v2g_params {
     double tau;                  /* TAU_PARAMETER (plugin.cc:97)          */
     double r_convergence;        /* R_CONVERGENCE                         */
@@ -80,10 +99,14 @@ typedef struct v2g_params {
     int maxiter;                 /* MAXITER                               */
     int cg_maxiter;              /* CG_MAXITER                            */
     int mu_update_frequency;     /* MU_UPDATE_FREQUENCY                   */
-    int precond;                 /* as v2g_cg_solve                       */
+    int precond;                 /* as v2g_cg_solve, or 3: learned online  */
     int flexible;
     int verbose;                 /* 1: one line per outer iteration       */
     double max_seconds;          /* > 0: stop after this wall time        */
+    int train_iters;             /* precond 3: outer iterations of data   */
+    int train_batch;             /* precond 3: rows sampled per iteration */
+    int train_steps;             /* precond 3: Adam steps per iteration   */
+    double train_lr;             /* precond 3: Adam learning rate         */
 } v2g_params;
 
 typedef struct v2g_stats {
@@ -93,6 +116,11 @@ typedef struct v2g_stats {
     double energy_primal, energy_dual, ep, ed, mu;
     double t_cg, t_xz, t_total;  /* seconds: CG phase, Update_xz, loop    */
     int eig_sweeps;              /* block-Jacobi sweeps, last Update_xz   */
+    int precond_active;          /* preconditioner of the last CG solve   */
+    long long cg_iters_train;    /* CG iterations during the training phase */
+    int train_outer;             /* outer iterations in the training phase */
+    double mlp_loss;             /* last training loss                     */
+    int fallback;                /* 1 if the learned PC was abandoned      */
 } v2g_stats;
 
 /* v2RDMSolver::compute_energy outer loop (v2rdm_solver.cc:1778-1912, orbital

@@ -25,7 +25,9 @@ class Params(ctypes.Structure):
                 ("maxiter", ctypes.c_int), ("cg_maxiter", ctypes.c_int),
                 ("mu_update_frequency", ctypes.c_int), ("precond", ctypes.c_int),
                 ("flexible", ctypes.c_int), ("verbose", ctypes.c_int),
-                ("max_seconds", ctypes.c_double)]
+                ("max_seconds", ctypes.c_double), ("train_iters", ctypes.c_int),
+                ("train_batch", ctypes.c_int), ("train_steps", ctypes.c_int),
+                ("train_lr", ctypes.c_double)]
 
 
 class Stats(ctypes.Structure):
@@ -34,7 +36,9 @@ class Stats(ctypes.Structure):
                 ("energy_dual", ctypes.c_double), ("ep", ctypes.c_double),
                 ("ed", ctypes.c_double), ("mu", ctypes.c_double), ("t_cg", ctypes.c_double),
                 ("t_xz", ctypes.c_double), ("t_total", ctypes.c_double),
-                ("eig_sweeps", ctypes.c_int)]
+                ("eig_sweeps", ctypes.c_int), ("precond_active", ctypes.c_int),
+                ("cg_iters_train", ctypes.c_longlong), ("train_outer", ctypes.c_int),
+                ("mlp_loss", ctypes.c_double), ("fallback", ctypes.c_int)]
 
 
 ITER_CALLBACK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
@@ -64,6 +68,14 @@ SIGNATURES = {
     "v2g_build_b": (ctypes.c_int, [_VP, _DP]),
     "v2g_build_c": (ctypes.c_int, [_VP, _DP, _DP, _DP]),
     "v2g_guess_random": (ctypes.c_int, [_VP, _DP, _DP, _DP]),
+    "v2g_mlp_num_params": (ctypes.c_int, []),
+    "v2g_mlp_init": (ctypes.c_int, [_VP, ctypes.c_ulonglong]),
+    "v2g_mlp_get_weights": (ctypes.c_int, [_VP, _DP]),
+    "v2g_mlp_set_weights": (ctypes.c_int, [_VP, _DP]),
+    "v2g_mlp_loss_grad": (ctypes.c_int, [_VP, ctypes.c_int, _DP, _DP, _DP, _DP,
+                                         ctypes.POINTER(ctypes.c_double)]),
+    "v2g_mlp_adam_step": (ctypes.c_int, [_VP, _DP, ctypes.c_double]),
+    "v2g_mlp_minv": (ctypes.c_int, [_VP, _DP, _DP, _DP]),
     "v2g_last_error": (ctypes.c_char_p, []),
     "v2g_version": (ctypes.c_char_p, []),
 }

@@ -18,6 +18,9 @@
 #include "eig.hpp"
 #include "kernels.hpp"
 #include "layout.hpp"
+#include "mlp.hpp"
+
+#include <random>
 
 namespace {
 
@@ -88,6 +91,15 @@ struct v2g_solver {
     int last_sweeps = 0;
     v2g_iter_callback cb = nullptr;
     void *cb_user = nullptr;
+    // learned preconditioner
+    DevBuf<double> rownorm;
+    bool have_rownorm = false;
+    DevBuf<unsigned char> code;
+    DevBuf<float> W, mlp_part, mlp_lpart, feat, tgt, wt;
+    DevBuf<double> am, av, grad, dloss, scales, yprev, r0, dy, lminv, vtmp;
+    DevBuf<int64_t> bidx;
+    int adam_t = 0, batch_cap = 0;
+    bool mlp_ready = false;
 
     explicit v2g_solver(const v2g_layout_desc &d);
     ~v2g_solver()
@@ -95,8 +107,13 @@ struct v2g_solver {
         if (own) (void)hipStreamDestroy(own);
     }
     void ensure_jacobi();
+    void ensure_mlp();
+    void mlp_init(unsigned long long seed);
+    void mlp_scales(const double *rhs, const double *y);
+    void mlp_infer(const double *rhs, const double *y);
+    double mlp_train(const double *rhs, int nb, int steps, double lr, std::mt19937_64 &rng);
     int cg(const double *rhs, double *y, double tol, int maxiter, int precond, int flexible,
-           double *res);
+           double *res, double *r0_out = nullptr);
     void update_xz(double mu, const double *y, const double *c, double *x, double *z);
     double dot(const double *a, const double *b, int64_t n);
 };
@@ -242,15 +259,105 @@ v2g_solver::v2g_solver(const v2g_layout_desc &d)
 void v2g_solver::ensure_jacobi()
 {
     if (have_jacobi) return;
+    rownorm.alloc(L->ncon);
     jacobi.alloc(L->ncon);
-    v2g::launch_row_norm2(plan, jacobi.p, st);
+    v2g::launch_row_norm2(plan, rownorm.p, st);
     std::vector<double> d(L->ncon);
-    HIPCHK(hipMemcpyAsync(d.data(), jacobi.p, sizeof(double) * d.size(), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(d.data(), rownorm.p, sizeof(double) * d.size(), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
     for (auto &v : d) v = v > 0.0 ? 1.0 / v : 1.0;
     HIPCHK(hipMemcpyAsync(jacobi.p, d.data(), sizeof(double) * d.size(), hipMemcpyHostToDevice, st));
     HIPCHK(hipStreamSynchronize(st));
     have_jacobi = true;
+    have_rownorm = true;
+}
+
+void v2g_solver::ensure_mlp()
+{
+    if (mlp_ready) return;
+    ensure_jacobi();
+    const int64_t n = L->ncon;
+    code.alloc(n);
+    v2g::mlp_family_codes(d_rowsegs.p, L->rowsegs, code.p, st);
+    W.alloc(v2g::kMlpParams);
+    am.alloc(v2g::kMlpParams);
+    av.alloc(v2g::kMlpParams);
+    grad.alloc(v2g::kMlpParams);
+    dloss.alloc(1);
+    scales.alloc(4);
+    yprev.alloc(n);
+    r0.alloc(n);
+    dy.alloc(n);
+    lminv.alloc(n);
+    vtmp.alloc(n);
+    mlp_ready = true;
+    mlp_init(12345);
+}
+
+void v2g_solver::mlp_init(unsigned long long seed)
+{
+    std::mt19937_64 rng(seed);
+    std::normal_distribution<double> nd(0.0, 1.0);
+    std::vector<float> w(v2g::kMlpParams, 0.0f);
+    const int in = v2g::kMlpIn, hd = v2g::kMlpH;
+    for (int i = 0; i < in * hd; i++) w[i] = (float)(nd(rng) / std::sqrt((double)in));
+    const int oW2 = in * hd + hd;
+    for (int i = 0; i < hd * hd; i++) w[oW2 + i] = (float)(nd(rng) / std::sqrt((double)hd));
+    // output layer (w3, b3) stays zero: g = 0, the preconditioner starts as Jacobi
+    HIPCHK(hipMemcpyAsync(W.p, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemsetAsync(am.p, 0, sizeof(double) * v2g::kMlpParams, st));
+    HIPCHK(hipMemsetAsync(av.p, 0, sizeof(double) * v2g::kMlpParams, st));
+    HIPCHK(hipStreamSynchronize(st));
+    adam_t = 0;
+}
+
+// scales[0] = 1/rms(rhs), scales[1] = 1/rms(y)
+void v2g_solver::mlp_scales(const double *rhs, const double *y)
+{
+    const double n = (double)L->ncon;
+    const double rr = dot(rhs, rhs, L->ncon), yy = dot(y, y, L->ncon);
+    double sc[2] = {rr > 0 ? std::sqrt(n / rr) : 1.0, yy > 0 ? std::sqrt(n / yy) : 1.0};
+    HIPCHK(hipMemcpyAsync(scales.p, sc, sizeof(sc), hipMemcpyHostToDevice, st));
+}
+
+void v2g_solver::mlp_infer(const double *rhs, const double *y)
+{
+    mlp_scales(rhs, y);
+    v2g::mlp_minv(L->ncon, code.p, rownorm.p, rhs, y, W.p, scales.p, lminv.p, 3.0, st);
+}
+
+// one batch of (r0, dy) pairs from the last Jacobi solve -> `steps` Adam steps
+double v2g_solver::mlp_train(const double *rhs, int nb, int steps, double lr, std::mt19937_64 &rng)
+{
+    if (nb > batch_cap) {
+        bidx.alloc(nb);
+        feat.alloc((size_t)nb * v2g::kMlpIn);
+        tgt.alloc(nb);
+        wt.alloc(nb);
+        mlp_part.alloc((size_t)v2g::mlp_grad_blocks(nb) * v2g::kMlpParams);
+        mlp_lpart.alloc(v2g::mlp_grad_blocks(nb));
+        batch_cap = nb;
+    }
+    std::vector<int64_t> idx(nb);
+    std::uniform_int_distribution<int64_t> ud(0, L->ncon - 1);
+    for (auto &i : idx) i = ud(rng);
+    HIPCHK(hipMemcpyAsync(bidx.p, idx.data(), sizeof(int64_t) * nb, hipMemcpyHostToDevice, st));
+    mlp_scales(rhs, yprev.p);
+    // weight normalization: 1 / mean (r0 / d)^2 over all rows
+    v2g::vec_mul(L->ncon, r0.p, jacobi.p, vtmp.p, st);
+    const double ss = dot(vtmp.p, vtmp.p, L->ncon);
+    const double wn = ss > 0 ? (double)L->ncon / ss : 1.0;
+    HIPCHK(hipMemcpyAsync(scales.p + 2, &wn, sizeof(double), hipMemcpyHostToDevice, st));
+    v2g::mlp_batch(nb, bidx.p, code.p, rownorm.p, rhs, yprev.p, r0.p, dy.p, scales.p, feat.p, tgt.p,
+                   wt.p, 3.0, st);
+    double loss = 0.0;
+    for (int k = 0; k < steps; k++) {
+        v2g::mlp_grad(nb, feat.p, tgt.p, wt.p, W.p, mlp_part.p, mlp_lpart.p, grad.p, dloss.p, st);
+        v2g::mlp_adam(W.p, grad.p, am.p, av.p, lr, ++adam_t, st);
+    }
+    HIPCHK(hipMemcpyAsync(&loss, dloss.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    return loss;
 }
 
 double v2g_solver::dot(const double *a, const double *b, int64_t n)
@@ -264,7 +371,7 @@ double v2g_solver::dot(const double *a, const double *b, int64_t n)
 }
 
 int v2g_solver::cg(const double *rhs, double *y, double tolv, int maxiter, int precond,
-                   int flexible, double *res)
+                   int flexible, double *res, double *r0_out)
 {
     const int64_t n = L->ncon;
     const double *minv = nullptr;
@@ -272,12 +379,16 @@ int v2g_solver::cg(const double *rhs, double *y, double tolv, int maxiter, int p
     else if (precond == 2) {
         if (!have_user) throw std::invalid_argument("precond 2 needs v2g_set_precond_diag");
         minv = user_minv.p;
+    } else if (precond == 4) {
+        minv = lminv.p;  // the learned diagonal from mlp_infer
     }
     double *zz = minv ? z.p : nullptr;
     // Ap = A A^T y ; r = b - Ap ; p = M^-1 r   (cg_solver.cc:141-150)
     v2g::launch_atu(plan, y, t.p, st);
     v2g::launch_au(plan, t.p, Ap.p, st);
     v2g::cg_init(n, rhs, Ap.p, r.p, p.p, minv, zz, partial.p, scal.p, tolv, maxiter, st);
+    if (r0_out)
+        HIPCHK(hipMemcpyAsync(r0_out, r.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     const int chunk = 8;
     double hs[v2g::CG_NSLOTS];
     int it = 0;
@@ -515,6 +626,13 @@ int v2g_bpsdp(v2g_handle h, const v2g_params *prm, const double *c, const double
     const double t0 = now();
     v2g_stats s{};
     bool stop = false;
+    const bool learned = prm->precond == 3;
+    if (learned) h->ensure_mlp();
+    std::mt19937_64 rng(20240607ULL);
+    long long cg_train = 0;
+    int n_train = 0;
+    bool fallback = false;
+    double last_loss = 0.0;
     do {
         const double t1 = now();
         // Ax = -tau mu (A x - b); z = c - z; B = A z + Ax  (v2rdm_solver.cc:1784-1794)
@@ -528,8 +646,35 @@ int v2g_bpsdp(v2g_handle h, const v2g_params *prm, const double *c, const double
         else cgtol = (ep > ed) ? 0.01 * ed : 0.01 * ep;
         if (cgtol < prm->cg_convergence) cgtol = prm->cg_convergence;
         double res = 0.0;
-        const int iiter = h->cg(h->Bv.p, y, cgtol, prm->cg_maxiter, prm->precond, prm->flexible, &res);
+        int pc = prm->precond, flex = prm->flexible;
+        const bool training = learned && oiter < prm->train_iters;
+        if (learned) {
+            if (training || fallback) {
+                pc = 1;
+                flex = 0;
+            } else {
+                h->mlp_infer(h->Bv.p, y);
+                pc = 4;
+                flex = 1;
+            }
+        }
+        if (training)
+            HIPCHK(hipMemcpyAsync(h->yprev.p, y, sizeof(double) * ncon, hipMemcpyDeviceToDevice, st));
+        const int iiter = h->cg(h->Bv.p, y, cgtol, prm->cg_maxiter, pc, flex, &res,
+                                training ? h->r0.p : nullptr);
         cgtot += iiter;
+        if (training) {
+            v2g::vec_axpby(ncon, 1.0, y, -1.0, h->yprev.p, h->dy.p, st);
+            last_loss = h->mlp_train(h->Bv.p, prm->train_batch > 0 ? prm->train_batch : 16384,
+                                     prm->train_steps > 0 ? prm->train_steps : 20,
+                                     prm->train_lr > 0 ? prm->train_lr : 1e-3, rng);
+            cg_train += iiter;
+            n_train++;
+        } else if (pc == 4 && (iiter >= prm->cg_maxiter || !std::isfinite(res))) {
+            // the true residual stayed above tolerance: keep Jacobi from here on
+            fallback = true;
+        }
+        s.precond_active = pc == 4 ? 3 : pc;
         const double t2 = now();
         tcg += t2 - t1;
         h->update_xz(mu, y, c, x, z);
@@ -563,6 +708,10 @@ int v2g_bpsdp(v2g_handle h, const v2g_params *prm, const double *c, const double
         s.t_xz = txz;
         s.t_total = now() - t0;
         s.eig_sweeps = h->last_sweeps;
+        s.cg_iters_train = cg_train;
+        s.train_outer = n_train;
+        s.mlp_loss = last_loss;
+        s.fallback = fallback ? 1 : 0;
         if (h->cb && h->cb(h->cb_user, oiter, &s)) stop = true;
         if (oiter == prm->maxiter) break;
         if (prm->max_seconds > 0 && now() - t0 > prm->max_seconds) break;
@@ -663,6 +812,78 @@ int v2g_guess_random(v2g_handle h, double *x, double *z, double *y)
         z[i] = ((double)rand() / RAND_MAX - 1.0) * 2.0;
     }
     for (int64_t i = 0; i < L.ncon; i++) y[i] = ((double)rand() / RAND_MAX - 1.0) * 2.0;
+    API_END
+}
+
+int v2g_mlp_num_params(void) { return v2g::kMlpParams; }
+
+int v2g_mlp_init(v2g_handle h, unsigned long long seed)
+{
+    API_BEGIN
+    if (!h) throw std::invalid_argument("null handle");
+    h->ensure_mlp();
+    h->mlp_init(seed);
+    API_END
+}
+
+int v2g_mlp_get_weights(v2g_handle h, float *w)
+{
+    API_BEGIN
+    if (!h || !w) throw std::invalid_argument("null argument");
+    h->ensure_mlp();
+    HIPCHK(hipMemcpyAsync(w, h->W.p, sizeof(float) * v2g::kMlpParams, hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    API_END
+}
+
+int v2g_mlp_set_weights(v2g_handle h, const float *w)
+{
+    API_BEGIN
+    if (!h || !w) throw std::invalid_argument("null argument");
+    h->ensure_mlp();
+    HIPCHK(hipMemcpyAsync(h->W.p, w, sizeof(float) * v2g::kMlpParams, hipMemcpyHostToDevice, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    API_END
+}
+
+int v2g_mlp_loss_grad(v2g_handle h, int nb, const float *feat, const float *tgt, const float *wt,
+                      double *grad, double *loss)
+{
+    API_BEGIN
+    if (!h || !feat || !tgt || !wt || !grad || nb <= 0) throw std::invalid_argument("bad argument");
+    h->ensure_mlp();
+    DevBuf<float> part, lpart;
+    part.alloc((size_t)v2g::mlp_grad_blocks(nb) * v2g::kMlpParams);
+    lpart.alloc(v2g::mlp_grad_blocks(nb));
+    v2g::mlp_grad(nb, feat, tgt, wt, h->W.p, part.p, lpart.p, h->grad.p, h->dloss.p, h->st);
+    HIPCHK(hipMemcpyAsync(grad, h->grad.p, sizeof(double) * v2g::kMlpParams,
+                          hipMemcpyDeviceToDevice, h->st));
+    double l = 0.0;
+    HIPCHK(hipMemcpyAsync(&l, h->dloss.p, sizeof(double), hipMemcpyDeviceToHost, h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
+    if (loss) *loss = l;
+    API_END
+}
+
+int v2g_mlp_adam_step(v2g_handle h, const double *grad, double lr)
+{
+    API_BEGIN
+    if (!h || !grad) throw std::invalid_argument("null argument");
+    h->ensure_mlp();
+    v2g::mlp_adam(h->W.p, grad, h->am.p, h->av.p, lr, ++h->adam_t, h->st);
+    HIPCHK(hipStreamSynchronize(h->st));
+    API_END
+}
+
+int v2g_mlp_minv(v2g_handle h, const double *rhs, const double *y, double *minv)
+{
+    API_BEGIN
+    if (!h || !rhs || !y || !minv) throw std::invalid_argument("null argument");
+    h->ensure_mlp();
+    h->mlp_infer(rhs, y);
+    HIPCHK(hipMemcpyAsync(minv, h->lminv.p, sizeof(double) * h->L->ncon, hipMemcpyDeviceToDevice,
+                          h->st));
+    HIPCHK(hipStreamSynchronize(h->st));
     API_END
 }
 

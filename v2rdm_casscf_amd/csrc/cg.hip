@@ -205,6 +205,14 @@ k_dotp(int64_t n, const double *__restrict__ a, const double *__restrict__ b,
     if (threadIdx.x == 0) partial[blockIdx.x] = t;
 }
 
+__global__ void __launch_bounds__(kThreads)
+k_mul(int64_t n, const double *a, const double *b, double *out)
+{
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kThreads)
+        out[i] = a[i] * b[i];
+}
+
 __global__ void k_reduce(const double *__restrict__ partial, double *__restrict__ out)
 {
     __shared__ double sh[kThreads / 64];
@@ -255,6 +263,11 @@ void vec_axpby(int64_t n, double a, const double *x, double b, const double *y, 
 void vec_dot_partial(int64_t n, const double *a, const double *b, double *partial, hipStream_t st)
 {
     hipLaunchKernelGGL(k_dotp, dim3(kRedBlocks), dim3(kThreads), 0, st, n, a, b, partial);
+}
+
+void vec_mul(int64_t n, const double *a, const double *b, double *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mul, dim3(kRedBlocks), dim3(kThreads), 0, st, n, a, b, out);
 }
 
 void vec_reduce(const double *partial, double *out, hipStream_t st)

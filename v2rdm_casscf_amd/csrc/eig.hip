@@ -195,7 +195,7 @@ k_fro(const EigBlock *__restrict__ blocks, const double *__restrict__ Mw, double
     if (threadIdx.x == 0) {
         double t = 0.0;
         for (int i = 0; i < kT / 64; i++) t += red[i];
-        tol[blockIdx.x] = 1e-15 * sqrt(t) + 1e-300;
+        tol[blockIdx.x] = 2e-14 * sqrt(t) + 1e-300;
     }
 }
 

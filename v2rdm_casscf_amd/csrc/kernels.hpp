@@ -57,5 +57,6 @@ void vec_axpby(int64_t n, double a, const double *x, double b, const double *y, 
                hipStream_t st);
 void vec_dot_partial(int64_t n, const double *a, const double *b, double *partial, hipStream_t st);
 void vec_reduce(const double *partial, double *out, hipStream_t st);
+void vec_mul(int64_t n, const double *a, const double *b, double *out, hipStream_t st);
 
 }  // namespace v2g

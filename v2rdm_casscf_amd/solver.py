@@ -75,7 +75,9 @@ class V2RDMSolver:
         # configurations here use them
         self.options = dict(TAU_PARAMETER=1.0, R_CONVERGENCE=1e-6, E_CONVERGENCE=1e-6,
                             CG_CONVERGENCE=1e-9, MAXITER=10000, CG_MAXITER=10000,
-                            MU_UPDATE_FREQUENCY=500, PRECONDITIONER="NONE", FLEXIBLE_CG=False)
+                            MU_UPDATE_FREQUENCY=500, PRECONDITIONER="NONE", FLEXIBLE_CG=False,
+                            MLP_TRAIN_ITERS=30, MLP_TRAIN_BATCH=16384, MLP_TRAIN_STEPS=20,
+                            MLP_TRAIN_LR=1e-3)
         if options:
             self.options.update({k.upper(): v for k, v in options.items()})
         self.x = self.z = self.y = self.c = self.b = None
@@ -149,6 +151,42 @@ class V2RDMSolver:
         self._sync()
         check(self.lib.v2g_set_precond_diag(self.h, _ptr(minv)))
 
+    # ---- learned preconditioner (no reference counterpart) ---------------
+    def mlp_num_params(self):
+        return self.lib.v2g_mlp_num_params()
+
+    def mlp_init(self, seed=12345):
+        check(self.lib.v2g_mlp_init(self.h, int(seed)))
+
+    def mlp_weights(self):
+        w = np.empty(self.mlp_num_params(), dtype=np.float32)
+        check(self.lib.v2g_mlp_get_weights(self.h, _np_ptr(w)))
+        return w
+
+    def mlp_set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        check(self.lib.v2g_mlp_set_weights(self.h, _np_ptr(w)))
+
+    def mlp_loss_grad(self, feat, tgt, wt):
+        """feat: (nb, 14) float32, tgt/wt: (nb,) float32 device tensors."""
+        nb = feat.shape[0]
+        g = torch.empty(self.mlp_num_params(), dtype=torch.float64, device=self.device)
+        loss = ctypes.c_double()
+        self._sync()
+        check(self.lib.v2g_mlp_loss_grad(self.h, int(nb), _ptr(feat.contiguous()),
+                                         _ptr(tgt.contiguous()), _ptr(wt.contiguous()), _ptr(g),
+                                         ctypes.byref(loss)))
+        return loss.value, g
+
+    def mlp_adam_step(self, grad, lr):
+        check(self.lib.v2g_mlp_adam_step(self.h, _ptr(grad), float(lr)))
+
+    def mlp_minv(self, rhs, y):
+        out = self._vec(self.nconstraints)
+        self._sync()
+        check(self.lib.v2g_mlp_minv(self.h, _ptr(rhs), _ptr(y), _ptr(out)))
+        return out
+
     # ---- Update_xz ------------------------------------------------------
     def update_xz(self, mu, y, c, x, z):
         for t, n, nm in ((y, self.nconstraints, "y"), (c, self.dimx, "c"), (x, self.dimx, "x"),
@@ -218,11 +256,13 @@ class V2RDMSolver:
         if guess or self.x is None:
             self.Guess()
         o = self.options
-        pc = {"NONE": 0, "JACOBI": 1, "LEARNED": 2}[str(o["PRECONDITIONER"]).upper()]
+        pc = {"NONE": 0, "JACOBI": 1, "LEARNED": 3}[str(o["PRECONDITIONER"]).upper()]
         p = _lib.Params(o["TAU_PARAMETER"], o["R_CONVERGENCE"], o["E_CONVERGENCE"],
                         o["CG_CONVERGENCE"], int(o["MAXITER"]), int(o["CG_MAXITER"]),
                         int(o["MU_UPDATE_FREQUENCY"]), pc, int(bool(o["FLEXIBLE_CG"])),
-                        int(bool(verbose)), float(max_seconds))
+                        int(bool(verbose)), float(max_seconds), int(o["MLP_TRAIN_ITERS"]),
+                        int(o["MLP_TRAIN_BATCH"]), int(o["MLP_TRAIN_STEPS"]),
+                        float(o["MLP_TRAIN_LR"]))
         st = _lib.Stats()
         self._sync()
         check(self.lib.v2g_bpsdp(self.h, ctypes.byref(p), _ptr(self.c), _ptr(self.b),
