@@ -82,7 +82,7 @@ struct v2g_solver {
     // eigen split
     std::vector<v2g::EigBlock> small, large;
     DevBuf<v2g::EigBlock> d_small, d_large;
-    DevBuf<double> Mw, Aw, Tw, Vw, Qbuf, tol;
+    DevBuf<double> Mw, Aw, Tw, Vw, Qbuf, tol, frop;
     DevBuf<int> active, counter;
     DevBuf<v2g::GemmDesc> d_gemm_warm1, d_gemm_warm2, d_gemm_rec;
     int max_pairs = 0, max_nb = 0, max_N = 0;
@@ -239,7 +239,7 @@ v2g_solver::v2g_solver(const v2g_layout_desc &d)
         max_pairs = std::max(max_pairs, np);
         max_nb = std::max(max_nb, e.N / 32);
         max_N = std::max(max_N, e.N);
-        max_tiles = std::max(max_tiles, 2 * (int64_t)np * np);
+        max_tiles = std::max(max_tiles, (int64_t)np * (np + 1) / 2 + (int64_t)np * np);
         large.push_back(e);
     }
     d_small.upload(small);
@@ -252,6 +252,7 @@ v2g_solver::v2g_solver(const v2g_layout_desc &d)
         Qbuf.alloc(qoff);
         active.alloc(poff);
         tol.alloc(large.size());
+        frop.alloc(64 * large.size());
         counter.alloc(1);
     }
 }
@@ -420,7 +421,7 @@ void v2g_solver::update_xz(double mu, const double *y, const double *c, double *
     const int nl = (int)large.size();
     const int cold = warm ? 0 : 1;
     v2g::eig_build_m(d_large.p, nl, ATy.p, c, x, mu, Mw.p, Aw.p, Vw.p, cold, st);
-    v2g::eig_tol(d_large.p, nl, Mw.p, tol.p, st);
+    v2g::eig_tol(d_large.p, nl, Mw.p, tol.p, frop.p, st);
     if (!cold) {
         // A = V^T (M V)
         if (!d_gemm_warm1.p) {

@@ -28,6 +28,8 @@ constexpr int kT = 256;      // threads per workgroup
 constexpr int kP = 64;       // pair sub-problem size
 constexpr int kLd = 65;      // LDS row pitch (breaks bank conflicts on columns)
 
+typedef double v4d __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ void circle_pair(int m, int r, int k, int &a, int &b)
 {
     // round r of the circle method on m players (m even): k = 0 pairs the
@@ -177,14 +179,18 @@ k_build_m(const EigBlock *__restrict__ blocks, const double *__restrict__ aty,
     }
 }
 
+constexpr int kFroParts = 64;
+
+// partial squared Frobenius norms: block y, part x
 __global__ void __launch_bounds__(kT)
-k_fro(const EigBlock *__restrict__ blocks, const double *__restrict__ Mw, double *__restrict__ tol)
+k_fro_part(const EigBlock *__restrict__ blocks, const double *__restrict__ Mw,
+           double *__restrict__ part)
 {
     __shared__ double red[kT / 64];
-    const EigBlock B = blocks[blockIdx.x];
+    const EigBlock B = blocks[blockIdx.y];
     const int64_t NN = (int64_t)B.N * B.N;
     double s = 0.0;
-    for (int64_t w = threadIdx.x; w < NN; w += kT) {
+    for (int64_t w = blockIdx.x * (int64_t)kT + threadIdx.x; w < NN; w += (int64_t)kFroParts * kT) {
         const double v = Mw[B.woff + w];
         s += v * v;
     }
@@ -195,8 +201,17 @@ k_fro(const EigBlock *__restrict__ blocks, const double *__restrict__ Mw, double
     if (threadIdx.x == 0) {
         double t = 0.0;
         for (int i = 0; i < kT / 64; i++) t += red[i];
-        tol[blockIdx.x] = 2e-14 * sqrt(t) + 1e-300;
+        part[blockIdx.y * kFroParts + blockIdx.x] = t;
     }
+}
+
+__global__ void k_fro_fin(int nblocks, const double *__restrict__ part, double *__restrict__ tol)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= nblocks) return;
+    double t = 0.0;
+    for (int i = 0; i < kFroParts; i++) t += part[b * kFroParts + i];
+    tol[b] = 2e-14 * sqrt(t) + 1e-300;
 }
 
 // index of the 64 pair rows: first column block a (32 indices), then b
@@ -221,13 +236,26 @@ k_pair_eig(const EigBlock *__restrict__ blocks, int round, const double *__restr
     int a, b;
     circle_pair(nbk, round, k, a, b);
     const double *A = Aw + B.woff;
+    const double tl = tol[blockIdx.y];
+    if (threadIdx.x == 0) sh_cnt = 0;
+    __syncthreads();
+    int big = 0;
     for (int w = threadIdx.x; w < kP * kP; w += kT) {
         const int i = w / kP, j = w % kP;
-        S[i * kLd + j] = A[(int64_t)pair_index(a, b, i) * B.N + pair_index(a, b, j)];
+        const double v = A[(int64_t)pair_index(a, b, i) * B.N + pair_index(a, b, j)];
+        S[i * kLd + j] = v;
         Q[i * kLd + j] = (i == j) ? 1.0 : 0.0;
+        big |= (i != j && fabs(v) > tl);
     }
+    if (big) atomicOr(&sh_cnt, 1);
     __syncthreads();
-    const int rot = lds_jacobi(S, Q, tol[blockIdx.y], 40, &sh_cnt, sh_c, sh_s, sh_pa, sh_pb);
+    const int any_big = sh_cnt;  // uniform across the workgroup
+    __syncthreads();             // every thread has read sh_cnt before it is reused
+    if (any_big == 0) {          // already diagonal to tolerance: identity rotation
+        if (threadIdx.x == 0) active[B.poff + k] = 0;
+        return;
+    }
+    const int rot = lds_jacobi(S, Q, tl, 12, &sh_cnt, sh_c, sh_s, sh_pa, sh_pb);
     double *Qo = Qbuf + B.qoff + (int64_t)k * kP * kP;
     for (int w = threadIdx.x; w < kP * kP; w += kT) Qo[w] = Q[(w / kP) * kLd + (w % kP)];
     if (threadIdx.x == 0) {
@@ -236,7 +264,45 @@ k_pair_eig(const EigBlock *__restrict__ blocks, int round, const double *__restr
     }
 }
 
-// A_PR <- Q_P^T A_PR Q_R for every pair tile; V[:, P] <- V[:, P] Q_P
+// 64x64x64 product on FP64 MFMA: out = op(X) Y with X, Y in LDS (pitch kLd);
+// transX: X is read transposed.  Wave w owns the 32x32 output sub-tile
+// (rows 32 (w/2), cols 32 (w%2)); v_mfma_f64_16x16x4f64 operand layout:
+// A[i][k] from lane i + 16k, B[k][j] from lane j + 16k, D[i][j] in lane
+// j + 16 (i/4), element i%4.
+__device__ __forceinline__ void mfma_tile(const double *X, const double *Y, bool transX,
+                                          v4d (&acc)[2][2])
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    const int li = lane & 15, lk = lane >> 4;
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int k0 = 0; k0 < kP; k0 += 4) {
+        const int k = k0 + lk;
+        double av[2], bv[2];
+        for (int a = 0; a < 2; a++) {
+            const int row = wr + a * 16 + li;
+            av[a] = transX ? X[k * kLd + row] : X[row * kLd + k];
+        }
+        for (int b = 0; b < 2; b++) bv[b] = Y[k * kLd + wc + b * 16 + li];
+        for (int a = 0; a < 2; a++)
+            for (int b = 0; b < 2; b++)
+                acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
+}
+
+__device__ __forceinline__ void store_tile(double *T, const v4d (&acc)[2][2])
+{
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+            for (int r = 0; r < 4; r++)
+                T[(wr + a * 16 + 4 * (lane >> 4) + r) * kLd + wc + b * 16 + (lane & 15)] = acc[a][b][r];
+}
+
+// A_PR <- Q_P^T A_PR Q_R for the pair tiles with P <= R (A is symmetric: the
+// (R, P) tile is written as the transpose); V[:, R] <- V[:, R] Q_R
 __global__ void __launch_bounds__(kT)
 k_apply(const EigBlock *__restrict__ blocks, int round, double *__restrict__ Aw,
         double *__restrict__ Vw, const double *__restrict__ Qbuf, const int *__restrict__ active)
@@ -246,13 +312,21 @@ k_apply(const EigBlock *__restrict__ blocks, int round, double *__restrict__ Aw,
     const EigBlock B = blocks[blockIdx.y];
     const int nbk = B.N / 32, np = nbk / 2;
     if (round >= nbk - 1) return;
-    const int64_t ntA = (int64_t)np * np, ntV = (int64_t)(B.N / kP) * np;
+    const int64_t ntA = (int64_t)np * (np + 1) / 2, ntV = (int64_t)(B.N / kP) * np;
     const int64_t t = blockIdx.x;
     if (t >= ntA + ntV) return;
     const bool isA = t < ntA;
-    int P, R, rt = 0;
-    if (isA) { P = (int)(t / np); R = (int)(t % np); }
-    else { rt = (int)((t - ntA) / np); R = (int)((t - ntA) % np); P = -1; }
+    int P = -1, R, rt = 0;
+    if (isA) {
+        // t -> (P, R) with P <= R, row-major over the upper triangle
+        int64_t rem = t;
+        P = 0;
+        while (rem >= np - P) { rem -= np - P; P++; }
+        R = P + (int)rem;
+    } else {
+        rt = (int)((t - ntA) / np);
+        R = (int)((t - ntA) % np);
+    }
     const bool actP = isA && active[B.poff + P];
     const bool actR = active[B.poff + R];
     if (!actP && !actR) return;
@@ -265,48 +339,36 @@ k_apply(const EigBlock *__restrict__ blocks, int round, double *__restrict__ Aw,
         const int i = w / kP, j = w % kP;
         T[i * kLd + j] = M[row_of(i) * B.N + pair_index(ra, rb, j)];
     }
-    const int ti = (threadIdx.x / 16) * 4, tj = (threadIdx.x % 16) * 4;  // 4x4 outputs
-    double acc[4][4];
+    v4d acc[2][2];
     if (actP) {
         const double *QP = Qbuf + B.qoff + (int64_t)P * kP * kP;
         for (int w = threadIdx.x; w < kP * kP; w += kT) Qs[(w / kP) * kLd + (w % kP)] = QP[w];
         __syncthreads();
-        // acc = Q_P^T T
-        for (int u = 0; u < 4; u++)
-            for (int v = 0; v < 4; v++) acc[u][v] = 0.0;
-        for (int kk = 0; kk < kP; kk++) {
-            double qa[4], tb[4];
-            for (int u = 0; u < 4; u++) qa[u] = Qs[kk * kLd + ti + u];
-            for (int v = 0; v < 4; v++) tb[v] = T[kk * kLd + tj + v];
-            for (int u = 0; u < 4; u++)
-                for (int v = 0; v < 4; v++) acc[u][v] += qa[u] * tb[v];
-        }
+        mfma_tile(Qs, T, true, acc);      // Q_P^T T
         __syncthreads();
-        for (int u = 0; u < 4; u++)
-            for (int v = 0; v < 4; v++) T[(ti + u) * kLd + tj + v] = acc[u][v];
+        store_tile(T, acc);
     }
     if (actR) {
         const double *QR = Qbuf + B.qoff + (int64_t)R * kP * kP;
         __syncthreads();
         for (int w = threadIdx.x; w < kP * kP; w += kT) Qs[(w / kP) * kLd + (w % kP)] = QR[w];
         __syncthreads();
-        for (int u = 0; u < 4; u++)
-            for (int v = 0; v < 4; v++) acc[u][v] = 0.0;
-        for (int kk = 0; kk < kP; kk++) {
-            double ta[4], qb[4];
-            for (int u = 0; u < 4; u++) ta[u] = T[(ti + u) * kLd + kk];
-            for (int v = 0; v < 4; v++) qb[v] = Qs[kk * kLd + tj + v];
-            for (int u = 0; u < 4; u++)
-                for (int v = 0; v < 4; v++) acc[u][v] += ta[u] * qb[v];
-        }
-    } else {
+        mfma_tile(T, Qs, false, acc);     // T Q_R
         __syncthreads();
-        for (int u = 0; u < 4; u++)
-            for (int v = 0; v < 4; v++) acc[u][v] = T[(ti + u) * kLd + tj + v];
+        store_tile(T, acc);
     }
-    for (int u = 0; u < 4; u++)
-        for (int v = 0; v < 4; v++)
-            M[row_of(ti + u) * B.N + pair_index(ra, rb, tj + v)] = acc[u][v];
+    __syncthreads();
+    for (int w = threadIdx.x; w < kP * kP; w += kT) {
+        const int i = w / kP, j = w % kP;
+        M[row_of(i) * B.N + pair_index(ra, rb, j)] = T[i * kLd + j];
+    }
+    if (isA && P != R) {
+        // (R, P) tile = transpose
+        for (int w = threadIdx.x; w < kP * kP; w += kT) {
+            const int j = w / kP, i = w % kP;
+            M[(int64_t)pair_index(ra, rb, j) * B.N + pair_index(pa, pb, i)] = T[i * kLd + j];
+        }
+    }
 }
 
 // T[:, j] = V[:, j] sqrt(max(lambda_j, 0) / mu), lambda_j = A[j][j]
@@ -353,9 +415,11 @@ void eig_build_m(const EigBlock *blocks, int nblocks, const double *aty, const d
                        Aw, Vw, cold);
 }
 
-void eig_tol(const EigBlock *blocks, int nblocks, const double *Mw, double *tol, hipStream_t st)
+void eig_tol(const EigBlock *blocks, int nblocks, const double *Mw, double *tol, double *part,
+             hipStream_t st)
 {
-    hipLaunchKernelGGL(k_fro, dim3(nblocks), dim3(kT), 0, st, blocks, Mw, tol);
+    hipLaunchKernelGGL(k_fro_part, dim3(kFroParts, nblocks), dim3(kT), 0, st, blocks, Mw, part);
+    hipLaunchKernelGGL(k_fro_fin, dim3((nblocks + 63) / 64), dim3(64), 0, st, nblocks, part, tol);
 }
 
 void eig_round(const EigBlock *blocks, int nblocks, int maxpairs, int64_t maxtiles, int round,

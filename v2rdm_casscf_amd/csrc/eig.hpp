@@ -30,7 +30,9 @@ void eig_small(const EigBlock *blocks, int nblocks, const double *aty, const dou
 void eig_build_m(const EigBlock *blocks, int nblocks, const double *aty, const double *c,
                  const double *x, double mu, double *Mw, double *Aw, double *Vw, int cold,
                  hipStream_t st);
-void eig_tol(const EigBlock *blocks, int nblocks, const double *Mw, double *tol, hipStream_t st);
+// tol[b] = 2e-14 ||M_b||_F; part holds 64 * nblocks doubles
+void eig_tol(const EigBlock *blocks, int nblocks, const double *Mw, double *tol, double *part,
+             hipStream_t st);
 void eig_round(const EigBlock *blocks, int nblocks, int maxpairs, int64_t maxtiles, int round,
                double *Aw, double *Vw, double *Qbuf, int *active, const double *tol, int *counter,
                hipStream_t st);

@@ -1,7 +1,9 @@
 // gemm.hip -- batched FP64 GEMM C = op(A) op(B) for the eigen split
 // (warm-start transform V^T M V and the reconstruction x = W W^T).
-// 64 x 64 output tile per workgroup, 16-deep K panels staged in LDS,
-// 4 x 4 outputs per thread.
+// 64 x 64 output tile per workgroup, 16-deep K panels staged in LDS; each
+// wave computes a 32 x 32 sub-tile with v_mfma_f64_16x16x4f64 (A[i][k] from
+// lane i + 16k, B[k][j] from lane j + 16k, D[i][j] in lane j + 16 (i/4),
+// element i % 4).
 #include <hip/hip_runtime.h>
 
 #include "eig.hpp"
@@ -11,6 +13,7 @@ namespace v2g {
 namespace {
 
 constexpr int kT = 256, kTile = 64, kK = 16;
+typedef double v4d __attribute__((ext_vector_type(4)));
 
 __global__ void __launch_bounds__(kT)
 k_gemm(const GemmDesc *__restrict__ descs)
@@ -22,8 +25,11 @@ k_gemm(const GemmDesc *__restrict__ descs)
     const int tiles_m = (d.M + kTile - 1) / kTile;
     if ((int)blockIdx.x >= tiles_m * tiles_n) return;
     const int m0 = (blockIdx.x / tiles_n) * kTile, n0 = (blockIdx.x % tiles_n) * kTile;
-    const int ti = (threadIdx.x / 16) * 4, tj = (threadIdx.x % 16) * 4;
-    double acc[4][4] = {};
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32, li = lane & 15, lk = lane >> 4;
+    v4d acc[2][2];
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
     for (int k0 = 0; k0 < d.K; k0 += kK) {
         for (int w = threadIdx.x; w < kK * kTile; w += kT) {
             int kk, r;
@@ -44,27 +50,27 @@ k_gemm(const GemmDesc *__restrict__ descs)
         }
         __syncthreads();
 #pragma unroll
-        for (int kk = 0; kk < kK; kk++) {
-            double a[4], b[4];
+        for (int k4 = 0; k4 < kK; k4 += 4) {
+            const int kk = k4 + lk;
+            double av[2], bv[2];
 #pragma unroll
-            for (int u = 0; u < 4; u++) a[u] = As[kk][ti + u];
+            for (int a = 0; a < 2; a++) av[a] = As[kk][wr + a * 16 + li];
 #pragma unroll
-            for (int v = 0; v < 4; v++) b[v] = Bs[kk][tj + v];
+            for (int b = 0; b < 2; b++) bv[b] = Bs[kk][wc + b * 16 + li];
 #pragma unroll
-            for (int u = 0; u < 4; u++)
+            for (int a = 0; a < 2; a++)
 #pragma unroll
-                for (int v = 0; v < 4; v++) acc[u][v] += a[u] * b[v];
+                for (int b = 0; b < 2; b++)
+                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
         }
         __syncthreads();
     }
-    for (int u = 0; u < 4; u++) {
-        const int gm = m0 + ti + u;
-        if (gm >= d.M) continue;
-        for (int v = 0; v < 4; v++) {
-            const int gn = n0 + tj + v;
-            if (gn < d.N) d.C[(int64_t)gm * d.ldc + gn] = acc[u][v];
-        }
-    }
+    for (int a = 0; a < 2; a++)
+        for (int b = 0; b < 2; b++)
+            for (int r = 0; r < 4; r++) {
+                const int gm = m0 + wr + a * 16 + 4 * lk + r, gn = n0 + wc + b * 16 + li;
+                if (gm < d.M && gn < d.N) d.C[(int64_t)gm * d.ldc + gn] = acc[a][b][r];
+            }
 }
 
 }  // namespace
