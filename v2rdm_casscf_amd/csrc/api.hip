@@ -75,7 +75,7 @@ struct v2g_solver {
     DevBuf<int64_t> d_rowtiles, d_xtiles, d_t2tiles;
     v2g::MapPlan plan;
     // CG
-    DevBuf<double> r, p, z, Ap, t, partial, scal, jacobi, user_minv;
+    DevBuf<double> r, p, z, Ap, t, partial, scal, jacobi, user_minv, rowpart;
     bool have_jacobi = false, have_user = false;
     // loop scratch
     DevBuf<double> Ax, Bv, ATy, red;
@@ -96,7 +96,7 @@ struct v2g_solver {
     bool have_rownorm = false;
     DevBuf<unsigned char> code;
     DevBuf<float> W, mlp_part, mlp_lpart, feat, tgt, wt;
-    DevBuf<double> am, av, grad, dloss, scales, yprev, r0, dy, lminv, vtmp;
+    DevBuf<double> am, av, grad, dloss, scales, yprev, r0, dy, lminv, vtmp, kd;
     DevBuf<int64_t> bidx;
     int adam_t = 0, batch_cap = 0;
     bool mlp_ready = false;
@@ -111,7 +111,8 @@ struct v2g_solver {
     void mlp_init(unsigned long long seed);
     void mlp_scales(const double *rhs, const double *y);
     void mlp_infer(const double *rhs, const double *y);
-    double mlp_train(const double *rhs, int nb, int steps, double lr, std::mt19937_64 &rng);
+    double mlp_train(const double *rhs, const double *rsample, int nb, int steps, double lr,
+                     std::mt19937_64 &rng);
     int cg(const double *rhs, double *y, double tol, int maxiter, int precond, int flexible,
            double *res, double *r0_out = nullptr);
     void update_xz(double mu, const double *y, const double *c, double *x, double *z);
@@ -205,10 +206,11 @@ v2g_solver::v2g_solver(const v2g_layout_desc &d)
     const int64_t ncon = L->ncon, dimx = L->dimx;
     r.alloc(ncon);
     p.alloc(ncon);
-    z.alloc(ncon);
+
     Ap.alloc(ncon);
     t.alloc(dimx);
     partial.alloc(v2g::kPartialArrays * v2g::kRedBlocks);
+    rowpart.alloc(std::max<int64_t>(1, plan.row_tiles));
     scal.alloc(v2g::CG_NSLOTS);
     Ax.alloc(ncon);
     Bv.alloc(ncon);
@@ -327,9 +329,16 @@ void v2g_solver::mlp_infer(const double *rhs, const double *y)
     v2g::mlp_minv(L->ncon, code.p, rownorm.p, rhs, y, W.p, scales.p, lminv.p, 3.0, st);
 }
 
-// one batch of (r0, dy) pairs from the last Jacobi solve -> `steps` Adam steps
-double v2g_solver::mlp_train(const double *rhs, int nb, int steps, double lr, std::mt19937_64 &rng)
+// training on residual samples of the Jacobi-CG solves: maximize the
+// step-quality J(s) = (r.Sr)^2 / ((Sr).A A^T (Sr)) of the learned diagonal
+// S = diag(s) (scale-invariant in s; Jacobi is the starting point).  Each
+// step: inference on every row, one A A^T product, two dots, a sampled
+// batch with the exact upstream gradient of -log J, backprop and Adam.
+// Returns log(J_learned / J_jacobi) on the last sample.
+double v2g_solver::mlp_train(const double *rhs, const double *rsample, int nb, int steps,
+                             double lr, std::mt19937_64 &rng)
 {
+    const int64_t n = L->ncon;
     if (nb > batch_cap) {
         bidx.alloc(nb);
         feat.alloc((size_t)nb * v2g::kMlpIn);
@@ -339,26 +348,35 @@ double v2g_solver::mlp_train(const double *rhs, int nb, int steps, double lr, st
         mlp_lpart.alloc(v2g::mlp_grad_blocks(nb));
         batch_cap = nb;
     }
+    if (!kd.p) kd.alloc(n);
+    auto jval = [&](const double *sv, double &a, double &b) {
+        v2g::vec_mul(n, sv, rsample, vtmp.p, st);
+        v2g::launch_atu(plan, vtmp.p, t.p, st);
+        v2g::launch_au(plan, t.p, kd.p, st);
+        a = dot(rsample, vtmp.p, n);
+        b = dot(vtmp.p, kd.p, n);
+        return (b > 0.0) ? a * a / b : 0.0;
+    };
+    double aj, bj;
+    const double jjac = jval(jacobi.p, aj, bj);
+    std::uniform_int_distribution<int64_t> ud(0, n - 1);
     std::vector<int64_t> idx(nb);
-    std::uniform_int_distribution<int64_t> ud(0, L->ncon - 1);
-    for (auto &i : idx) i = ud(rng);
-    HIPCHK(hipMemcpyAsync(bidx.p, idx.data(), sizeof(int64_t) * nb, hipMemcpyHostToDevice, st));
-    mlp_scales(rhs, yprev.p);
-    // weight normalization: 1 / mean (r0 / d)^2 over all rows
-    v2g::vec_mul(L->ncon, r0.p, jacobi.p, vtmp.p, st);
-    const double ss = dot(vtmp.p, vtmp.p, L->ncon);
-    const double wn = ss > 0 ? (double)L->ncon / ss : 1.0;
-    HIPCHK(hipMemcpyAsync(scales.p + 2, &wn, sizeof(double), hipMemcpyHostToDevice, st));
-    v2g::mlp_batch(nb, bidx.p, code.p, rownorm.p, rhs, yprev.p, r0.p, dy.p, scales.p, feat.p, tgt.p,
-                   wt.p, 3.0, st);
-    double loss = 0.0;
+    double jl = jjac;
     for (int k = 0; k < steps; k++) {
-        v2g::mlp_grad(nb, feat.p, tgt.p, wt.p, W.p, mlp_part.p, mlp_lpart.p, grad.p, dloss.p, st);
+        mlp_infer(rhs, yprev.p);              // s = lminv, scales[0..1]
+        double a, b;
+        jl = jval(lminv.p, a, b);
+        double ab[2] = {a, b};
+        HIPCHK(hipMemcpyAsync(scales.p + 2, ab, sizeof(ab), hipMemcpyHostToDevice, st));
+        for (auto &i : idx) i = ud(rng);
+        HIPCHK(hipMemcpyAsync(bidx.p, idx.data(), sizeof(int64_t) * nb, hipMemcpyHostToDevice, st));
+        v2g::mlp_batch_j(nb, bidx.p, code.p, rownorm.p, rhs, yprev.p, rsample, lminv.p, kd.p,
+                         scales.p, feat.p, wt.p, 3.0, (double)n / nb, st);
+        v2g::mlp_grad(nb, feat.p, nullptr, wt.p, W.p, mlp_part.p, mlp_lpart.p, grad.p, dloss.p, st);
         v2g::mlp_adam(W.p, grad.p, am.p, av.p, lr, ++adam_t, st);
     }
-    HIPCHK(hipMemcpyAsync(&loss, dloss.p, sizeof(double), hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
-    return loss;
+    return (jjac > 0.0 && jl > 0.0) ? std::log(jl / jjac) : 0.0;
 }
 
 double v2g_solver::dot(const double *a, const double *b, int64_t n)
@@ -383,11 +401,10 @@ int v2g_solver::cg(const double *rhs, double *y, double tolv, int maxiter, int p
     } else if (precond == 4) {
         minv = lminv.p;  // the learned diagonal from mlp_infer
     }
-    double *zz = minv ? z.p : nullptr;
     // Ap = A A^T y ; r = b - Ap ; p = M^-1 r   (cg_solver.cc:141-150)
     v2g::launch_atu(plan, y, t.p, st);
     v2g::launch_au(plan, t.p, Ap.p, st);
-    v2g::cg_init(n, rhs, Ap.p, r.p, p.p, minv, zz, partial.p, scal.p, tolv, maxiter, st);
+    v2g::cg_init(n, rhs, Ap.p, r.p, p.p, minv, nullptr, partial.p, scal.p, tolv, maxiter, st);
     if (r0_out)
         HIPCHK(hipMemcpyAsync(r0_out, r.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     const int chunk = 8;
@@ -398,10 +415,10 @@ int v2g_solver::cg(const double *rhs, double *y, double tolv, int maxiter, int p
             const int par = it & 1;
             const double *done = scal.p + v2g::CG_DONE0 + par;
             v2g::launch_atu(plan, p.p, t.p, st, done);
-            v2g::launch_au(plan, t.p, Ap.p, st, done);
-            v2g::cg_dot(n, p.p, Ap.p, partial.p, scal.p, par, st);
-            v2g::cg_step(n, y, r.p, p.p, Ap.p, minv, zz, nullptr, partial.p, scal.p, par, st);
-            v2g::cg_finish_iter(n, r.p, p.p, zz, partial.p, scal.p, par, flexible, it, st);
+            v2g::launch_au(plan, t.p, Ap.p, st, done, p.p, rowpart.p);   // Ap and p.Ap
+            v2g::cg_dot_from_rows(plan.row_tiles, rowpart.p, partial.p, scal.p, par, st);
+            v2g::cg_step(n, y, r.p, p.p, Ap.p, minv, nullptr, nullptr, partial.p, scal.p, par, st);
+            v2g::cg_finish_iter(n, r.p, p.p, minv, partial.p, scal.p, par, flexible, it, st);
         }
         HIPCHK(hipMemcpyAsync(hs, scal.p, sizeof(hs), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
@@ -632,7 +649,8 @@ int v2g_bpsdp(v2g_handle h, const v2g_params *prm, const double *c, const double
     std::mt19937_64 rng(20240607ULL);
     long long cg_train = 0;
     int n_train = 0;
-    bool fallback = false;
+    bool fallback = false, first_learned = true;
+    int last_jacobi_iters = 0;
     double last_loss = 0.0;
     do {
         const double t1 = now();
@@ -665,15 +683,22 @@ int v2g_bpsdp(v2g_handle h, const v2g_params *prm, const double *c, const double
                                 training ? h->r0.p : nullptr);
         cgtot += iiter;
         if (training) {
-            v2g::vec_axpby(ncon, 1.0, y, -1.0, h->yprev.p, h->dy.p, st);
-            last_loss = h->mlp_train(h->Bv.p, prm->train_batch > 0 ? prm->train_batch : 16384,
-                                     prm->train_steps > 0 ? prm->train_steps : 20,
-                                     prm->train_lr > 0 ? prm->train_lr : 1e-3, rng);
+            // residual samples: the solve's initial residual and its final one
+            // (the slowest modes of the Jacobi-preconditioned operator)
+            const double *rs = (oiter & 1) ? h->r.p : h->r0.p;
+            last_loss = h->mlp_train(h->Bv.p, rs, prm->train_batch > 0 ? prm->train_batch : 16384,
+                                     prm->train_steps > 0 ? prm->train_steps : 10,
+                                     prm->train_lr > 0 ? prm->train_lr : 2e-3, rng);
             cg_train += iiter;
             n_train++;
-        } else if (pc == 4 && (iiter >= prm->cg_maxiter || !std::isfinite(res))) {
-            // the true residual stayed above tolerance: keep Jacobi from here on
-            fallback = true;
+            last_jacobi_iters = iiter;
+        } else if (pc == 4) {
+            // the true residual is the judge: abandon the learned diagonal if it
+            // failed to converge, or if its first solve needed clearly more
+            // iterations than the last Jacobi solve
+            if (iiter >= prm->cg_maxiter || !std::isfinite(res)) fallback = true;
+            if (first_learned && iiter > 1.15 * last_jacobi_iters + 2) fallback = true;
+            first_learned = false;
         }
         s.precond_active = pc == 4 ? 3 : pc;
         const double t2 = now();

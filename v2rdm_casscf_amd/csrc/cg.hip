@@ -129,7 +129,6 @@ k_step(int64_t n, double *__restrict__ y, double *__restrict__ r, const double *
         const double rn = ro - alpha * Ap[i];
         r[i] = rn;
         const double zn = minv ? minv[i] * rn : rn;
-        if (z) z[i] = zn;
         rz += rn * zn;
         rr += rn * rn;
         roz += ro * zn;
@@ -146,7 +145,7 @@ k_step(int64_t n, double *__restrict__ y, double *__restrict__ r, const double *
     }
 }
 
-// convergence test and p = z + beta p
+// convergence test and p = M^-1 r + beta p (z carries M^-1 here, not z)
 __global__ void __launch_bounds__(kThreads)
 k_finish(int64_t n, double *__restrict__ p, const double *__restrict__ r,
          const double *__restrict__ z, double *__restrict__ partial, double *__restrict__ scal,
@@ -178,10 +177,24 @@ k_finish(int64_t n, double *__restrict__ p, const double *__restrict__ r,
     if (conv) return;
     const double beta = (rz - (flexible ? roz : 0.0)) / rzold;
     if (blockIdx.x == 0 && threadIdx.x == 0) scal[S_BETA] = beta;
-    const double *zz = z ? z : r;
     for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kThreads)
-        p[i] = zz[i] + beta * p[i];
+        p[i] = (z ? z[i] * r[i] : r[i]) + beta * p[i];
+}
+
+// partial[P_PAP][b] = sum of a contiguous chunk of the per-tile partials
+__global__ void __launch_bounds__(kThreads)
+k_rows_to_partials(int64_t ntiles, const double *__restrict__ rowpart, double *__restrict__ partial,
+                   const double *__restrict__ scal, int parity)
+{
+    __shared__ double sh[kThreads / 64];
+    if (scal[S_DONE0 + parity] != 0.0) return;
+    const int64_t chunk = (ntiles + gridDim.x - 1) / gridDim.x;
+    const int64_t lo = blockIdx.x * chunk, hi = lo + chunk < ntiles ? lo + chunk : ntiles;
+    double s = 0.0;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kThreads) s += rowpart[i];
+    const double t = block_sum(s, sh);
+    if (threadIdx.x == 0) partial[P_PAP * kRedBlocks + blockIdx.x] = t;
 }
 
 __global__ void __launch_bounds__(kThreads)
@@ -237,6 +250,13 @@ void cg_dot(int64_t n, const double *a, const double *b, double *partial, double
 {
     hipLaunchKernelGGL(k_dot, dim3(kRedBlocks), dim3(kThreads), 0, st, n, a, b, partial, scal,
                        parity);
+}
+
+void cg_dot_from_rows(int64_t ntiles, const double *rowpart, double *partial, const double *scal,
+                      int parity, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_rows_to_partials, dim3(kRedBlocks), dim3(kThreads), 0, st, ntiles, rowpart,
+                       partial, scal, parity);
 }
 
 void cg_step(int64_t n, double *y, double *r, double *p, const double *Ap, const double *minv,

@@ -27,8 +27,10 @@ struct MapPlan {
 };
 
 int64_t tiles_for(const Segment &s);
+// dotv / rowpart (optional): rowpart[tile] = sum over the tile's rows of dotv . Ax
 void launch_au(const MapPlan &p, const double *x, double *Ax, hipStream_t st,
-               const double *done = nullptr);
+               const double *done = nullptr, const double *dotv = nullptr,
+               double *rowpart = nullptr);
 void launch_atu(const MapPlan &p, const double *y, double *ATy, hipStream_t st,
                 const double *done = nullptr);
 void launch_row_norm2(const MapPlan &p, double *d, hipStream_t st);
@@ -47,6 +49,9 @@ void cg_init(int64_t n, const double *b, const double *Ap, double *r, double *p,
              double *z, double *partial, double *scal, double tol, int maxiter, hipStream_t st);
 void cg_dot(int64_t n, const double *a, const double *b, double *partial, double *scal, int parity,
             hipStream_t st);
+// p.Ap from the per-tile partials of the fused A.x launch
+void cg_dot_from_rows(int64_t ntiles, const double *rowpart, double *partial, const double *scal,
+                      int parity, hipStream_t st);
 void cg_step(int64_t n, double *y, double *r, double *p, const double *Ap, const double *minv,
              double *z, double *rprev, double *partial, double *scal, int parity, hipStream_t st);
 void cg_finish_iter(int64_t n, double *r, double *p, const double *z, double *partial, double *scal,

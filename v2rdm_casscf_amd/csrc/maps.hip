@@ -32,6 +32,8 @@ using namespace rows;
 
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / 64;
+constexpr int kPerThread = 4;            // independent elementwise items per thread
+constexpr int kElemTile = kBlock * kPerThread;
 
 __device__ __forceinline__ double wave_sum(double v)
 {
@@ -70,27 +72,57 @@ __device__ __forceinline__ int find_segment(const int64_t *__restrict__ tile_sta
 __global__ void __launch_bounds__(kBlock)
 k_au(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
      const int64_t *__restrict__ tile_start, int nseg, const double *__restrict__ x,
-     double *__restrict__ Ax, const double *__restrict__ done)
+     double *__restrict__ Ax, const double *__restrict__ done, const double *__restrict__ dotv,
+     double *__restrict__ rowpart)
 {
     if (done && *done != 0.0) return;
+    __shared__ double red[kWavesPerBlock];
     const DevLayout &L = *Lp;
     const int si = find_segment(tile_start, nseg, blockIdx.x);
     const Segment sg = segs[si];
     const int64_t t = blockIdx.x - tile_start[si];
+    double contrib = 0.0;
     if (sg.wave) {
         const int64_t item = t * kWavesPerBlock + (threadIdx.x >> 6);
-        if (item >= sg.count) return;
-        const int lane = threadIdx.x & 63;
-        Gather f{x, 0.0};
-        wave_terms(L, sg, item, lane, f);
-        const double v = wave_sum(f.acc);
-        if (lane == 0) Ax[sg.start + item] = v;
+        if (item < sg.count) {
+            const int lane = threadIdx.x & 63;
+            Gather f{x, 0.0};
+            wave_terms(L, sg, item, lane, f);
+            const double v = wave_sum(f.acc);
+            if (lane == 0) {
+                Ax[sg.start + item] = v;
+                if (dotv) contrib = v * dotv[sg.start + item];
+            }
+        }
     } else {
-        const int64_t e = t * kBlock + threadIdx.x;
-        if (e >= sg.count) return;
-        Gather f{x, 0.0};
-        elem_terms(L, sg, e, f);
-        Ax[sg.start + e] = f.acc;
+        // kPerThread independent rows per thread: their gathers overlap in flight
+        double v[kPerThread];
+#pragma unroll
+        for (int u = 0; u < kPerThread; u++) {
+            const int64_t e = t * kElemTile + u * kBlock + threadIdx.x;
+            Gather f{x, 0.0};
+            if (e < sg.count) elem_terms(L, sg, e, f);
+            v[u] = f.acc;
+        }
+#pragma unroll
+        for (int u = 0; u < kPerThread; u++) {
+            const int64_t e = t * kElemTile + u * kBlock + threadIdx.x;
+            if (e < sg.count) {
+                Ax[sg.start + e] = v[u];
+                if (dotv) contrib += v[u] * dotv[sg.start + e];
+            }
+        }
+    }
+    if (rowpart) {
+        // this workgroup's share of dotv . Ax, in a fixed order
+        contrib = wave_sum(contrib);
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = contrib;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double s2 = 0.0;
+            for (int w = 0; w < kWavesPerBlock; w++) s2 += red[w];
+            rowpart[blockIdx.x] = s2;
+        }
     }
 }
 
@@ -124,11 +156,13 @@ k_row_norm2(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
         }
         if (lane == 0) d[sg.start + item] = v;
     } else {
-        const int64_t e = t * kBlock + threadIdx.x;
-        if (e >= sg.count) return;
-        Collect f;
-        elem_terms(L, sg, e, f);
-        d[sg.start + e] = f.norm2();
+        for (int u = 0; u < kPerThread; u++) {
+            const int64_t e = t * kElemTile + u * kBlock + threadIdx.x;
+            if (e >= sg.count) return;
+            Collect f;
+            elem_terms(L, sg, e, f);
+            d[sg.start + e] = f.norm2();
+        }
     }
 }
 
@@ -150,15 +184,24 @@ k_atu(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
         if (lane == 0) ATy[sg.start + item] = v;
         return;
     }
-    const int64_t e = t * kBlock + threadIdx.x;
-    if (e >= sg.count) return;
-    double v;
-    switch (sg.kind) {
-    case XS_D2AB: v = atu_d2ab(L, sg, e, y); break;
-    case XS_D2AA: v = atu_d2aa(L, sg, e, y); break;
-    default: v = atu_self(L, sg, e, y); break;
+    double v[kPerThread];
+#pragma unroll
+    for (int u = 0; u < kPerThread; u++) {
+        const int64_t e = t * kElemTile + u * kBlock + threadIdx.x;
+        v[u] = 0.0;
+        if (e < sg.count) {
+            switch (sg.kind) {
+            case XS_D2AB: v[u] = atu_d2ab(L, sg, e, y); break;
+            case XS_D2AA: v[u] = atu_d2aa(L, sg, e, y); break;
+            default: v[u] = atu_self(L, sg, e, y); break;
+            }
+        }
     }
-    ATy[sg.start + e] = v;
+#pragma unroll
+    for (int u = 0; u < kPerThread; u++) {
+        const int64_t e = t * kElemTile + u * kBlock + threadIdx.x;
+        if (e < sg.count) ATy[sg.start + e] = v[u];
+    }
 }
 
 // T2 rows: self entry written, delta-selected D2 / D1 entries accumulated
@@ -171,21 +214,24 @@ k_atu_t2(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
     const DevLayout &L = *Lp;
     const int si = find_segment(tile_start, nseg, blockIdx.x);
     const Segment sg = segs[si];
-    const int64_t e = (blockIdx.x - tile_start[si]) * kBlock + threadIdx.x;
-    if (e >= sg.count) return;
-    const double yv = y[sg.aux + e];
-    ATy[sg.start + e] = -yv;
-    Scatter f{ATy, yv};
-    t2_terms(L, sg.kind, sg.sub, sg.h, e, f, false);
+    for (int u = 0; u < kPerThread; u++) {
+        const int64_t e = (blockIdx.x - tile_start[si]) * kElemTile + u * kBlock + threadIdx.x;
+        if (e >= sg.count) return;
+        const double yv = y[sg.aux + e];
+        ATy[sg.start + e] = -yv;
+        Scatter f{ATy, yv};
+        t2_terms(L, sg.kind, sg.sub, sg.h, e, f, false);
+    }
 }
 
 }  // namespace
 
-void launch_au(const MapPlan &p, const double *x, double *Ax, hipStream_t st, const double *done)
+void launch_au(const MapPlan &p, const double *x, double *Ax, hipStream_t st, const double *done,
+               const double *dotv, double *rowpart)
 {
     if (p.row_tiles > 0)
         hipLaunchKernelGGL(k_au, dim3((unsigned)p.row_tiles), dim3(kBlock), 0, st, p.dlay,
-                           p.rowsegs, p.row_tile_start, p.nrowsegs, x, Ax, done);
+                           p.rowsegs, p.row_tile_start, p.nrowsegs, x, Ax, done, dotv, rowpart);
 }
 
 void launch_row_norm2(const MapPlan &p, double *d, hipStream_t st)
@@ -207,7 +253,7 @@ void launch_atu(const MapPlan &p, const double *y, double *ATy, hipStream_t st, 
 
 int64_t tiles_for(const Segment &s)
 {
-    const int64_t per = s.wave ? kWavesPerBlock : kBlock;
+    const int64_t per = s.wave ? kWavesPerBlock : kElemTile;
     return (s.count + per - 1) / per;
 }
 

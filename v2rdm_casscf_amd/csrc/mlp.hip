@@ -130,6 +130,33 @@ k_mlp_batch(int nb, const int64_t *__restrict__ idx, const unsigned char *__rest
     wt[b] = (float)(step * step * scales[2]);
 }
 
+// batch for the step-quality objective J = (r.Sr)^2 / ((Sr).K(Sr)), S = diag(s):
+// features of the sampled rows and the upstream gradient of -log J with
+// respect to the network output g_i (s_i = exp(g_i) / d_i), zero where g is
+// clamped; `coef` = (ncon / nb) so the batch sum estimates the full sum
+__global__ void __launch_bounds__(kT)
+k_mlp_batch_j(int nb, const int64_t *__restrict__ idx, const unsigned char *__restrict__ code,
+              const double *__restrict__ d, const double *__restrict__ rhs,
+              const double *__restrict__ yprev, const double *__restrict__ r,
+              const double *__restrict__ s, const double *__restrict__ kd,
+              const double *__restrict__ scales, float *__restrict__ feat,
+              float *__restrict__ dout, double clampv, double coef)
+{
+    const int b = blockIdx.x * kT + threadIdx.x;
+    if (b >= nb) return;
+    const int64_t i = idx[b];
+    float f[kMlpIn];
+    features(code[i], d[i], rhs[i], yprev[i], (float)scales[0], (float)scales[1], f);
+    for (int k = 0; k < kMlpIn; k++) feat[(int64_t)b * kMlpIn + k] = f[k];
+    const double a = scales[2], bq = scales[3];
+    const double di = d[i] > 0.0 ? d[i] : 1.0;
+    const double g = log(s[i] * di);
+    double u = 0.0;
+    if (fabs(g) < clampv - 1e-6 && a != 0.0 && bq != 0.0)
+        u = -s[i] * (2.0 * r[i] * r[i] / a - 2.0 * kd[i] * r[i] / bq);
+    dout[b] = (float)(coef * u);
+}
+
 constexpr int kS = 128;     // samples per workgroup in the training kernel
 constexpr int kLdH = kMlpH + 1;
 
@@ -175,9 +202,13 @@ k_mlp_grad(int nb, const float *__restrict__ feat, const float *__restrict__ tgt
         }
         float dout = 0.0f;
         if (valid) {
-            const float r = o - tgt[s];
-            lossv = wt[s] * r * r;
-            dout = 2.0f * wt[s] * r / (float)nb;
+            if (tgt) {
+                const float r = o - tgt[s];
+                lossv = wt[s] * r * r;
+                dout = 2.0f * wt[s] * r / (float)nb;
+            } else {
+                dout = wt[s];  // upstream gradient d(loss)/d(out) given directly
+            }
         }
         DO[t] = dout;
     }
@@ -317,6 +348,15 @@ void mlp_batch(int nb, const int64_t *idx, const unsigned char *code, const doub
 {
     hipLaunchKernelGGL(k_mlp_batch, dim3((nb + kT - 1) / kT), dim3(kT), 0, st, nb, idx, code, d,
                        rhs, yprev, r0, dy, scales, feat, tgt, wt, clampv);
+}
+
+void mlp_batch_j(int nb, const int64_t *idx, const unsigned char *code, const double *d,
+                 const double *rhs, const double *yprev, const double *r, const double *s,
+                 const double *kd, const double *scales, float *feat, float *dout, double clampv,
+                 double coef, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_mlp_batch_j, dim3((nb + kT - 1) / kT), dim3(kT), 0, st, nb, idx, code, d,
+                       rhs, yprev, r, s, kd, scales, feat, dout, clampv, coef);
 }
 
 int mlp_grad_blocks(int nb) { return (nb + kS - 1) / kS; }

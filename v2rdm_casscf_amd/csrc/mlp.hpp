@@ -26,7 +26,13 @@ void mlp_batch(int nb, const int64_t *idx, const unsigned char *code, const doub
                const double *rhs, const double *yprev, const double *r0, const double *dy,
                const double *scales, float *feat, float *tgt, float *wt, double clampv,
                hipStream_t st);
+// scales[2] = a = r.Sr, scales[3] = b = (Sr).K(Sr)
+void mlp_batch_j(int nb, const int64_t *idx, const unsigned char *code, const double *d,
+                 const double *rhs, const double *yprev, const double *r, const double *s,
+                 const double *kd, const double *scales, float *feat, float *dout, double clampv,
+                 double coef, hipStream_t st);
 int mlp_grad_blocks(int nb);
+// tgt == nullptr: wt holds the upstream gradient d(loss)/d(out) per sample
 void mlp_grad(int nb, const float *feat, const float *tgt, const float *wt, const float *W,
               float *part, float *loss_part, double *grad, double *loss, hipStream_t st);
 void mlp_adam(float *W, const double *grad, double *m, double *v, double lr, int step,
