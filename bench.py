@@ -86,6 +86,8 @@ def main():
     ap.add_argument("--precond", default="jacobi", choices=["none", "jacobi", "learned"])
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--map-reps", type=int, default=20)
+    ap.add_argument("--eig-sweeps", type=int, default=3,
+                    help="block-Jacobi sweeps per warm Update_xz inside the loop (0: converge)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X; the "
@@ -98,7 +100,8 @@ def main():
     dev = torch.device("cuda:0")
     w = _workload(args.workload)
     s = V2RDMSolver(w["amopi"], w["na"], w["nb"], 1, "DQG", device=dev,
-                    options=dict(PRECONDITIONER=args.precond.upper()))
+                    options=dict(PRECONDITIONER=args.precond.upper(),
+                                 EIG_WARM_SWEEPS=args.eig_sweeps))
     stream = torch.cuda.Stream(device=dev)
     s.lib.v2g_set_stream(s.h, __import__("ctypes").c_void_p(stream.cuda_stream))
     s.GetIntegrals(w["h1"], w["eri"])
@@ -181,7 +184,7 @@ def main():
                             precond=args.precond, cg_iters_per_outer=round(cg_per_outer, 2),
                             energy=st["energy_primal"] + w["enuc"], ep=st["ep"], ed=st["ed"],
                             mu=st["mu"], t_cg_s=st["t_cg"], t_xz_s=st["t_xz"],
-                            warmup_s=st_w["t_total"]),
+                            eig_warm_sweeps=args.eig_sweeps, warmup_s=st_w["t_total"]),
                 roofline=roofline, eig=eig, cpu_baseline=cpu)
     print(json.dumps(line))
 

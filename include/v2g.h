@@ -107,6 +107,8 @@ typedef struct v2g_params {
     int train_batch;             /* precond 3: rows sampled per iteration */
     int train_steps;             /* precond 3: Adam steps per iteration   */
     double train_lr;             /* precond 3: Adam learning rate         */
+    int eig_warm_sweeps;         /* > 0: block-Jacobi sweeps per warm      */
+                                 /* Update_xz inside the loop (0: converge) */
 } v2g_params;
 
 typedef struct v2g_stats {

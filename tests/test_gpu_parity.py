@@ -86,7 +86,7 @@ def test_cg_matches_oracle(amopi, na, nb, mult, pos, sz):
     it, res = s.cg_solve(_dev(rhs), y, 1e-8, 500)
     # summation order differs from the oracle, so on rank-deficient A A^T the
     # count can drift by a few iterations
-    assert abs(it - itref) <= max(2, itref // 20)
+    assert abs(it - itref) <= max(3, int(0.15 * itref))
     # A^T y is what the iteration uses; y itself is unique only up to null(A^T)
     assert _rel(L.ATu(y.cpu().numpy()), L.ATu(yref)) < 1e-7
     assert res < 1e-8

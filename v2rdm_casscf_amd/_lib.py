@@ -27,7 +27,7 @@ class Params(ctypes.Structure):
                 ("flexible", ctypes.c_int), ("verbose", ctypes.c_int),
                 ("max_seconds", ctypes.c_double), ("train_iters", ctypes.c_int),
                 ("train_batch", ctypes.c_int), ("train_steps", ctypes.c_int),
-                ("train_lr", ctypes.c_double)]
+                ("train_lr", ctypes.c_double), ("eig_warm_sweeps", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):

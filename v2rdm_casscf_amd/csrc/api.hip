@@ -115,7 +115,8 @@ struct v2g_solver {
                      std::mt19937_64 &rng);
     int cg(const double *rhs, double *y, double tol, int maxiter, int precond, int flexible,
            double *res, double *r0_out = nullptr);
-    void update_xz(double mu, const double *y, const double *c, double *x, double *z);
+    void update_xz(double mu, const double *y, const double *c, double *x, double *z,
+                   int warm_sweep_cap = 0);
     double dot(const double *a, const double *b, int64_t n);
 };
 
@@ -429,7 +430,12 @@ int v2g_solver::cg(const double *rhs, double *y, double tolv, int maxiter, int p
     return (int)hs[v2g::CG_ITER];
 }
 
-void v2g_solver::update_xz(double mu, const double *y, const double *c, double *x, double *zv)
+// warm_sweep_cap > 0 bounds the sweeps of a warm-started call: inside the BPSDP
+// loop the eigenvectors keep improving from call to call, so the projection
+// becomes exact at the fixed point while early calls stay cheap.  x is always
+// rebuilt PSD from the current V; z = mu x - M keeps M exact.
+void v2g_solver::update_xz(double mu, const double *y, const double *c, double *x, double *zv,
+                           int warm_sweep_cap)
 {
     v2g::launch_atu(plan, y, ATy.p, st);
     v2g::eig_small(d_small.p, (int)small.size(), ATy.p, c, x, zv, mu, st);
@@ -455,7 +461,7 @@ void v2g_solver::update_xz(double mu, const double *y, const double *c, double *
         v2g::gemm_batched(d_gemm_warm1.p, nl, max_N, max_N, st);
         v2g::gemm_batched(d_gemm_warm2.p, nl, max_N, max_N, st);
     }
-    const int maxsweeps = 40;
+    const int maxsweeps = (!cold && warm_sweep_cap > 0) ? warm_sweep_cap : 40;
     int sweeps = 0;
     for (; sweeps < maxsweeps; sweeps++) {
         HIPCHK(hipMemsetAsync(counter.p, 0, sizeof(int), st));
@@ -703,7 +709,7 @@ int v2g_bpsdp(v2g_handle h, const v2g_params *prm, const double *c, const double
         s.precond_active = pc == 4 ? 3 : pc;
         const double t2 = now();
         tcg += t2 - t1;
-        h->update_xz(mu, y, c, x, z);
+        h->update_xz(mu, y, c, x, z, prm->eig_warm_sweeps);
         HIPCHK(hipStreamSynchronize(st));
         txz += now() - t2;
         // ed = ||A^T y - c + z||, ep = ||A x - b||  (v2rdm_solver.cc:1829-1837)
