@@ -30,6 +30,7 @@ typedef struct v2g_layout_desc {
     int multiplicity;      /* 2S+1                                           */
     int constrain_q2, constrain_g2, constrain_t2;
     int constrain_spin, constrain_sz;
+    int constrain_t1;      /* POSITIVITY DQGT1 / DQGT1T2 / DQGT              */
 } v2g_layout_desc;
 
 /* Replaces v2RDMSolver construction + common_init sizes (v2rdm_solver.cc:

@@ -12,7 +12,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "build", "liboracle_v2rdm.so")
 
 OFF = dict(D2AB=0, D2AA=1, D2BB=2, D200=3, D1A=4, D1B=5, Q1A=6, Q1B=7, Q2AB=8, Q2AA=9,
-           Q2BB=10, G2AB=11, G2BA=12, G2AA=13, T2AAA=14, T2BBB=15, T2AAB=16, T2BBA=17)
+           Q2BB=10, G2AB=11, G2BA=12, G2AA=13, T2AAA=14, T2BBB=15, T2AAB=16, T2BBA=17,
+           T1AAA=18, T1BBB=19, T1AAB=20, T1BBA=21)
 
 
 class _Options(ctypes.Structure):
@@ -20,7 +21,16 @@ class _Options(ctypes.Structure):
                 ("nalpha", ctypes.c_int), ("nbeta", ctypes.c_int),
                 ("multiplicity", ctypes.c_int), ("constrain_q2", ctypes.c_int),
                 ("constrain_g2", ctypes.c_int), ("constrain_t2", ctypes.c_int),
-                ("constrain_spin", ctypes.c_int), ("constrain_sz", ctypes.c_int)]
+                ("constrain_spin", ctypes.c_int), ("constrain_sz", ctypes.c_int),
+                ("constrain_t1", ctypes.c_int)]
+
+
+def positivity_flags(positivity):
+    """(q2, g2, t1, t2) of a POSITIVITY string, as plugin.cc / v2rdm_solver.cc:488-523."""
+    p = positivity.upper()
+    if p == "DQGT":
+        return 1, 1, 1, 1
+    return int("Q" in p), int("G" in p), int("T1" in p), int("T2" in p)
 
 
 class _Params(ctypes.Structure):
@@ -60,7 +70,7 @@ def lib():
         L.orc_block_dims.argtypes = [vp, ip]
         L.orc_offset.restype = ctypes.c_long
         L.orc_offset.argtypes = [vp, ctypes.c_int, ctypes.c_int]
-        for f in ("orc_gems_ab", "orc_gems_aa", "orc_trip_aab", "orc_trip_aba",
+        for f in ("orc_gems_ab", "orc_gems_aa", "orc_trip_aab", "orc_trip_aba", "orc_trip_aaa",
                   "orc_orbital_irrep"):
             getattr(L, f).restype = ctypes.c_int
             getattr(L, f).argtypes = [vp, ctypes.c_int]
@@ -118,9 +128,7 @@ class Layout:
         o.nalpha, o.nbeta = nalpha, nbeta
         o.multiplicity = multiplicity if multiplicity else (nalpha - nbeta + 1)
         p = positivity.upper()
-        o.constrain_q2 = int("Q" in p)
-        o.constrain_g2 = int("G" in p)
-        o.constrain_t2 = int("T2" in p)
+        o.constrain_q2, o.constrain_g2, o.constrain_t1, o.constrain_t2 = positivity_flags(p)
         o.constrain_spin = int(constrain_spin)
         o.constrain_sz = int(constrain_sz)
         self.opts = o
@@ -158,6 +166,9 @@ class Layout:
 
     def trip_aba(self, h):
         return lib().orc_trip_aba(self.h, h)
+
+    def trip_aaa(self, h):
+        return lib().orc_trip_aaa(self.h, h)
 
     def ibas_ab(self, p, q):
         return lib().orc_ibas_ab(self.h, p, q)

@@ -9,7 +9,8 @@
  *   layout sizes      dimx / ncon against the counts printed in the
  *                     reference's tests/v2rdm{1..8}/output.ref (golden).
  *   Au rows + b       exact-RDM feasibility: A.x(FCI RDMs) = b to 1e-12 for
- *                     brute-force FCI states (C1 and D2h, singlet and triplet).
+ *                     brute-force FCI states (C1 and D2h, singlet and triplet);
+ *                     T1/T2 blocks built physically as <B+B> + <BB+>.
  *   ATu               adjointness <A x, y> = <x, A^T y> against Au.
  *   c, energy         c.x(FCI RDMs) = E_FCI.
  *   CG                against a dense least-squares solve of A A^T y = rhs.
@@ -45,7 +46,7 @@ struct orc_layout {
     int gems_ab[8], gems_aa[8];
     int *bab[8], *baa[8];             /* pairs (first, second) */
     int trip_aab[8], trip_aba[8], trip_aaa[8];
-    int *baab[8], *baba[8];           /* triplets (i, j, k) */
+    int *baab[8], *baba[8], *baaa[8]; /* triplets (i, j, k) */
     long off[ORC_OFF_COUNT][8];
     long dimx, ncon;
     int nblocks;
@@ -117,11 +118,13 @@ orc_layout *orc_layout_create(const orc_options *opt)
         L->gems_ab[h] = nab;
         L->gems_aa[h] = naa;
     }
-    /* triplets: basis.cc:655-757 (only needed for T2) */
-    if (opt->constrain_t2) {
+    /* triplets: basis.cc:655-757 (T1 and T2); aba: all (i,j,k), aab: i < j,
+     * aaa: i < j < k, each in (i outer, k inner) order */
+    if (opt->constrain_t2 || opt->constrain_t1) {
         for (int h = 0; h < L->nirrep; h++) {
             L->baab[h] = (int *)malloc(sizeof(int) * 3 * (amo * amo * amo + 1));
             L->baba[h] = (int *)malloc(sizeof(int) * 3 * (amo * amo * amo + 1));
+            L->baaa[h] = (int *)malloc(sizeof(int) * 3 * (amo * amo * amo + 1));
             int caba = 0, caab = 0, caaa = 0;
             for (int i = 0; i < amo; i++)
                 for (int j = 0; j < amo; j++)
@@ -135,6 +138,8 @@ orc_layout *orc_layout_create(const orc_options *opt)
                         L->baab[h][3 * caab + 2] = k;
                         caab++;
                         if (j >= k) continue;
+                        L->baaa[h][3 * caaa] = i; L->baaa[h][3 * caaa + 1] = j;
+                        L->baaa[h][3 * caaa + 2] = k;
                         caaa++;
                     }
             L->trip_aba[h] = caba;
@@ -147,7 +152,7 @@ orc_layout *orc_layout_create(const orc_options *opt)
     /* x offsets in dimensions_ order, v2rdm_solver.cc:698-844 */
     long o = 0;
     int nb = 0;
-    int *bdim = (int *)malloc(sizeof(int) * 8 * 24);
+    int *bdim = (int *)malloc(sizeof(int) * 8 * ORC_OFF_COUNT);
 #define BLOCKS(w, dimexpr) for (int h = 0; h < L->nirrep; h++) { long d_ = (dimexpr); \
         L->off[w][h] = o; o += d_ * d_; bdim[nb++] = (int)d_; }
     BLOCKS(ORC_OFF_D2AB, g[h])
@@ -170,6 +175,12 @@ orc_layout *orc_layout_create(const orc_options *opt)
         BLOCKS(ORC_OFF_G2AB, g[h])
         BLOCKS(ORC_OFF_G2BA, g[h])
         BLOCKS(ORC_OFF_G2AA, 2 * g[h])
+    }
+    if (opt->constrain_t1) {             /* v2rdm_solver.cc:809-825 */
+        BLOCKS(ORC_OFF_T1AAA, L->trip_aaa[h])
+        BLOCKS(ORC_OFF_T1BBB, L->trip_aaa[h])
+        BLOCKS(ORC_OFF_T1AAB, L->trip_aab[h])
+        BLOCKS(ORC_OFF_T1BBA, L->trip_aab[h])
     }
     if (opt->constrain_t2) {
         BLOCKS(ORC_OFF_T2AAA, L->trip_aab[h] + L->trip_aba[h])
@@ -219,6 +230,17 @@ orc_layout *orc_layout_create(const orc_options *opt)
         add_family(L, "g2ba", &r, sab);
         add_family(L, "g2aa", &r, 4 * sab);
     }
+    if (opt->constrain_t1) {             /* t1.cc:433-809 row order: aab, bba, aaa, bbb */
+        long s1 = 0, s3 = 0;
+        for (int h = 0; h < L->nirrep; h++) {
+            s1 += (long)L->trip_aab[h] * L->trip_aab[h];
+            s3 += (long)L->trip_aaa[h] * L->trip_aaa[h];
+        }
+        add_family(L, "t1aab", &r, s1);
+        add_family(L, "t1bba", &r, s1);
+        add_family(L, "t1aaa", &r, s3);
+        add_family(L, "t1bbb", &r, s3);
+    }
     if (opt->constrain_t2) {
         long s1 = 0, s2 = 0;
         for (int h = 0; h < L->nirrep; h++) {
@@ -239,7 +261,7 @@ void orc_layout_destroy(orc_layout *L)
     if (!L) return;
     for (int h = 0; h < L->nirrep; h++) {
         free(L->bab[h]); free(L->baa[h]);
-        free(L->baab[h]); free(L->baba[h]);
+        free(L->baab[h]); free(L->baba[h]); free(L->baaa[h]);
     }
     free(L->gab); free(L->gaa); free(L->bdim); free(L->boff);
     free(L);
@@ -254,6 +276,7 @@ int orc_gems_ab(const orc_layout *L, int h) { return L->gems_ab[h]; }
 int orc_gems_aa(const orc_layout *L, int h) { return L->gems_aa[h]; }
 int orc_trip_aab(const orc_layout *L, int h) { return L->trip_aab[h]; }
 int orc_trip_aba(const orc_layout *L, int h) { return L->trip_aba[h]; }
+int orc_trip_aaa(const orc_layout *L, int h) { return L->trip_aaa[h]; }
 int orc_ibas_ab(const orc_layout *L, int p, int q) { return L->gab[p * L->amo + q]; }
 int orc_ibas_aa(const orc_layout *L, int p, int q) { return p == q ? -1 : L->gaa[p * L->amo + q]; }
 int orc_orbital_irrep(const orc_layout *L, int p) { return L->sym[p]; }
@@ -615,7 +638,7 @@ void orc_build_b(const orc_layout *L, double *b)
                 off += (long)g * g;
             }
     }
-    /* G2 and T2 rows have b = 0 */
+    /* G2, T1 and T2 rows have b = 0 (v2rdm_solver.cc:2821-2860) */
 }
 
 /* RepackIntegrals + FrozenCoreEnergy (no core), tei.cc:144-228, 229-313:

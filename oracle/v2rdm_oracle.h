@@ -14,6 +14,7 @@
  *   orc_Au / orc_ATu    v2rdm_solver.cc:3020-3065 / 3114-3158 dispatch over
  *                       d2.cc:296-547 / 58-293, spin.cc:274-493 / 58-271,
  *                       q2.cc:801-950 / 953-1074, g2.cc:1061-1260 / 1263-1457,
+ *                       t1.cc:433-809 / 812-1173,
  *                       t2.cc:1251-1861 / 3399-3993 (the *_slow maps)
  *   orc_cg_Ax           v2rdm_solver.cc:3207-3213
  *   orc_cg_solve        cg_solver.cc:125-177
@@ -39,6 +40,7 @@ typedef struct orc_options {
     int multiplicity;    /* 2S+1                                              */
     int constrain_q2, constrain_g2, constrain_t2;
     int constrain_spin, constrain_sz;
+    int constrain_t1;    /* POSITIVITY DQGT1 / DQGT1T2                        */
 } orc_options;
 
 typedef struct orc_layout orc_layout;
@@ -57,6 +59,7 @@ int  orc_gems_ab(const orc_layout *L, int h);
 int  orc_gems_aa(const orc_layout *L, int h);
 int  orc_trip_aab(const orc_layout *L, int h);
 int  orc_trip_aba(const orc_layout *L, int h);
+int  orc_trip_aaa(const orc_layout *L, int h);
 /* geminal tables: ab index of (p,q); aa index of (p,q) (-1 if p==q) */
 int  orc_ibas_ab(const orc_layout *L, int p, int q);
 int  orc_ibas_aa(const orc_layout *L, int p, int q);
@@ -72,6 +75,7 @@ enum {
     ORC_OFF_Q2AB, ORC_OFF_Q2AA, ORC_OFF_Q2BB,
     ORC_OFF_G2AB, ORC_OFF_G2BA, ORC_OFF_G2AA,
     ORC_OFF_T2AAA, ORC_OFF_T2BBB, ORC_OFF_T2AAB, ORC_OFF_T2BBA,
+    ORC_OFF_T1AAA, ORC_OFF_T1BBB, ORC_OFF_T1AAB, ORC_OFF_T1BBA,
     ORC_OFF_COUNT
 };
 

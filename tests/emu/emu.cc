@@ -32,6 +32,7 @@ struct Emu {
         D.q2 = o.q2;
         D.g2 = o.g2;
         D.t2 = o.t2;
+        D.t1 = o.t1;
         D.na = L.na;
         D.nb = L.nb;
         for (int h = 0; h < 8; h++) {
@@ -41,6 +42,8 @@ struct Emu {
             D.gems_aa[h] = L.gems_aa[h];
             D.trip_aab[h] = L.trip_aab[h];
             D.trip_aba[h] = L.trip_aba[h];
+            D.trip_aaa[h] = L.trip_aaa[h];
+            D.baaa_off[h] = L.baaa_off[h];
             D.bab_off[h] = L.bab_off[h];
             D.baa_off[h] = L.baa_off[h];
             D.baab_off[h] = L.baab_off[h];
@@ -55,6 +58,7 @@ struct Emu {
         D.baa = L.baa.data();
         D.baab = L.baab.data();
         D.baba = L.baba.data();
+        D.baaa = L.baaa.data();
     }
 };
 
@@ -85,7 +89,7 @@ struct CheckedScatter {
 extern "C" {
 
 void *emu_create(int nirrep, const int *amopi, int na, int nb, int mult, int q2, int g2, int t2,
-                 int spin, int sz)
+                 int spin, int sz, int t1)
 {
     Options o;
     o.nirrep = nirrep;
@@ -96,6 +100,7 @@ void *emu_create(int nirrep, const int *amopi, int na, int nb, int mult, int q2,
     o.q2 = q2;
     o.g2 = g2;
     o.t2 = t2;
+    o.t1 = t1;
     o.spin = spin;
     o.sz = sz;
     try {
@@ -179,7 +184,7 @@ void emu_ATu(void *h, const double *y, double *ATy)
             const double yv = y[sg.aux + e];
             ATy[i] = -yv;
             CheckedScatter f{ATy, dimx, &E->violations, yv};
-            rows::t2_terms(E->D, sg.kind, sg.sub, sg.h, e, f, false);
+            rows::triplet_terms(E->D, sg.kind, sg.sub, sg.h, e, f, false);
         }
     }
     for (int64_t i = 0; i < dimx; i++)

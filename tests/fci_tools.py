@@ -111,6 +111,8 @@ def exact_x(F, psi, L):
       Q2ab(ij,kl)= <a_ia a_jb a+_lb a+_ka>    : vectors a+_jb a+_ia psi
       G2ab(ij,kl)= <a+_ia a_jb a+_lb a_ka>    : vectors a+_jb a_ia psi
       G2 super block: vectors [a+_ja a_ia ; a+_jb a_ib] psi
+      T1 blocks  = <B+ B> + <B B+>, B_ijk = a_k a_j a_i (aab: i<j alpha, k beta;
+                   aaa: i<j<k), i.e. the D3 + Q3 matrix
       T2 blocks  = <B+ B> + <B B+> for B = the triplet operators of each block
     """
     n = L.amo
@@ -190,6 +192,26 @@ def exact_x(F, psi, L):
             if L.opts.constrain_q2:
                 vq = [c(q, s) @ (c(p, s) @ psi) for p, q in P]
                 put(qn, h, gram(vq))
+    def anti(ops):
+        """<B+ B> + <B B+> as Gram of B psi and B+ psi."""
+        v1 = [op @ psi for op in ops]
+        v2 = [op.T @ psi for op in ops]
+        return gram(v1) + gram(v2)
+
+    if getattr(L.opts, "constrain_t1", 0):
+        for h in range(L.nirrep):
+            trip = [(i, j, k) for i in range(n) for j in range(n) for k in range(n)
+                    if sym[i] ^ sym[j] ^ sym[k] == h]
+            aab = [t for t in trip if t[0] < t[1]]
+            aaa = [t for t in trip if t[0] < t[1] < t[2]]
+            for name, s1, s2 in (("T1AAB", 0, 1), ("T1BBA", 1, 0)):
+                ops = [a(k, s2) @ a(j, s1) @ a(i, s1) for i, j, k in aab]
+                if ops:
+                    put(name, h, anti(ops))
+            for name, s1 in (("T1AAA", 0), ("T1BBB", 1)):
+                ops = [a(k, s1) @ a(j, s1) @ a(i, s1) for i, j, k in aaa]
+                if ops:
+                    put(name, h, anti(ops))
     if L.opts.constrain_t2:
         for h in range(L.nirrep):
             aab, aba = [], []
@@ -201,12 +223,6 @@ def exact_x(F, psi, L):
                         aba.append((i, j, k))
                         if i < j:
                             aab.append((i, j, k))
-
-            def anti(ops):
-                """<B+ B> + <B B+> as Gram of B psi and B+ psi."""
-                v1 = [op @ psi for op in ops]
-                v2 = [op.T @ psi for op in ops]
-                return gram(v1) + gram(v2)
 
             # T2aab: B_ijk = a+_kb a_ja a_ia ; T2bba: spins swapped
             for name, s1, s2 in (("T2AAB", 0, 1), ("T2BBA", 1, 0)):

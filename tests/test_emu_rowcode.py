@@ -33,6 +33,11 @@ LAYOUTS = [
     ([3, 0, 1, 1, 0, 2, 1, 1], 4, 3, 2, "DQGT2", True),
     ([9], 4, 4, 1, "DQG", True),
     ([5, 0, 0, 0, 0, 5, 0, 0], 5, 5, 1, "DQG", True),
+    ([5], 3, 2, 2, "DQGT1", True),
+    ([6], 3, 3, 1, "DQGT1", True),
+    ([3, 0, 1, 1, 0, 2, 1, 1], 4, 3, 2, "DQGT1", True),
+    ([2, 1, 1, 1], 3, 2, 2, "DQGT1T2", True),
+    ([3, 2], 3, 3, 1, "DQGT", False),
 ]
 
 
@@ -42,7 +47,7 @@ def emu():
         subprocess.check_call(["make", "-C", os.path.join(HERE, "emu")])
     lib = ctypes.CDLL(EMU)
     lib.emu_create.restype = ctypes.c_void_p
-    lib.emu_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int)] + [ctypes.c_int] * 8
+    lib.emu_create.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int)] + [ctypes.c_int] * 9
     lib.emu_destroy.argtypes = [ctypes.c_void_p]
     lib.emu_dims.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_longlong),
                              ctypes.POINTER(ctypes.c_longlong)]
@@ -56,9 +61,8 @@ def emu():
 
 def _make(lib, amopi, na, nb, mult, pos, sz):
     a = (ctypes.c_int * 8)(*(amopi + [0] * (8 - len(amopi))))
-    p = pos.upper()
-    h = lib.emu_create(len(amopi), a, na, nb, mult, int("Q" in p), int("G" in p),
-                       int("T2" in p), 1, int(sz))
+    q2, g2, t1, t2 = orc.positivity_flags(pos)
+    h = lib.emu_create(len(amopi), a, na, nb, mult, q2, g2, t2, 1, int(sz), t1)
     assert h
     dx, nc = ctypes.c_longlong(), ctypes.c_longlong()
     lib.emu_dims(h, ctypes.byref(dx), ctypes.byref(nc))

@@ -27,6 +27,10 @@ LAYOUTS = [
     ([2, 1, 1, 1], 2, 2, 1, "DQGT2", True),
     ([9], 4, 4, 1, "DQG", True),
     ([5, 0, 0, 0, 0, 5, 0, 0], 5, 5, 1, "DQG", True),
+    ([5], 3, 2, 2, "DQGT1", True),
+    ([6], 3, 3, 1, "DQGT1", True),
+    ([3, 0, 1, 1, 0, 2, 1, 1], 4, 3, 2, "DQGT1", True),
+    ([2, 1, 1, 1], 3, 2, 2, "DQGT1T2", True),
 ]
 
 
@@ -128,3 +132,29 @@ def test_bpsdp_energy_matches_oracle(amopi, na, nb, pos):
     out = s.compute_energy(h1, eri)
     assert out["converged"] and ref["converged"]
     assert abs(out["energy_primal"] - ref["energy_primal"]) < 1e-6
+
+
+def _golden_energies():
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "bpsdp_energies.json")
+    return json.load(open(path))
+
+
+@pytest.mark.parametrize("pos", ["DQGT1", "DQGT2", "DQGT"])
+def test_bpsdp_energy_with_three_body_conditions(pos):
+    """Converged T1 / T2 energies against the oracle's (tests/golden), same
+    Hamiltonian and random guess, R/E_CONVERGENCE 1e-6."""
+    from v2rdm_casscf_amd import V2RDMSolver
+    import fci_tools as ft
+    cs = _golden_energies()[0]
+    if pos not in cs["energies"]:
+        pytest.skip(f"no golden energy for {pos}")
+    h1, eri = ft.random_hamiltonian(sum(cs["amopi"]), cs["seed"], None)
+    s = V2RDMSolver(cs["amopi"], cs["nalpha"], cs["nbeta"], None, pos,
+                    options=dict(MAXITER=40000))
+    assert (s.dimx, s.nconstraints) == (cs["energies"][pos]["dimx"], cs["energies"][pos]["ncon"])
+    out = s.compute_energy(h1, eri)
+    assert out["converged"]
+    assert abs(out["energy_primal"] - cs["energies"][pos]["energy"]) < 1e-6
+    assert out["energy_primal"] <= cs["fci"] + 1e-6

@@ -36,6 +36,10 @@ LAYOUTS = [
     ([4], 2, 2, 1, "DQGT2", True),
     ([5], 3, 2, 2, "DQGT2", True),
     ([2, 1, 1, 1], 2, 2, 1, "DQGT2", True),
+    ([4], 2, 2, 1, "DQGT1", True),
+    ([5], 3, 2, 2, "DQGT1", True),
+    ([3, 2], 3, 2, 2, "DQGT1", True),
+    ([2, 1, 1, 1], 3, 3, 1, "DQGT", True),
 ]
 
 
@@ -89,6 +93,24 @@ def test_feasibility_detects_a_planted_sign_error():
     g = L.gems_aa(0)
     x[o:o + g * g] *= -1.0
     assert np.abs(L.Au(x) - L.build_b()).max() > 1e-3
+
+
+def test_t1_feasibility_detects_planted_sign_errors():
+    """T1 rows read the mixed-spin G2 block (aab rows) and the G2 super block
+    (aaa rows): a sign slip in either shows up as T1 residuals."""
+    L = _layout([5], 3, 2, 2, "DQGT1")
+    h, eri = _ham(L)
+    F = ft.Fock(L.amo)
+    E, psi = F.ground_state(h, eri, 3, 2, s_target=0.5)
+    fam = {name: (st, cnt) for name, st, cnt in L.row_families()}
+    for blk, rows in (("G2BA", "t1aab"), ("G2AA", "t1aaa"), ("Q2BB", "t1bba")):
+        x = ft.exact_x(F, psi, L)
+        st, cnt = fam[rows]
+        assert np.abs((L.Au(x) - L.build_b())[st:st + cnt]).max() < 1e-11
+        o = L.offset(blk, 0)
+        g = {"G2BA": L.gems_ab(0), "G2AA": 2 * L.gems_ab(0), "Q2BB": L.gems_aa(0)}[blk]
+        x[o:o + g * g] *= -1.0
+        assert np.abs((L.Au(x) - L.build_b())[st:st + cnt]).max() > 1e-3, blk
 
 
 @pytest.mark.parametrize("amopi,na,nb,mult,pos,sz", LAYOUTS)

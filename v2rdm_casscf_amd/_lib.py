@@ -16,7 +16,8 @@ class LayoutDesc(ctypes.Structure):
                 ("nalpha", ctypes.c_int), ("nbeta", ctypes.c_int),
                 ("multiplicity", ctypes.c_int), ("constrain_q2", ctypes.c_int),
                 ("constrain_g2", ctypes.c_int), ("constrain_t2", ctypes.c_int),
-                ("constrain_spin", ctypes.c_int), ("constrain_sz", ctypes.c_int)]
+                ("constrain_spin", ctypes.c_int), ("constrain_sz", ctypes.c_int),
+                ("constrain_t1", ctypes.c_int)]
 
 
 class Params(ctypes.Structure):

@@ -69,7 +69,7 @@ struct v2g_solver {
     std::unique_ptr<v2g::Layout> L;
     hipStream_t own = nullptr, st = nullptr;
     // layout on device
-    DevBuf<int> d_sym, d_gab, d_gaa, d_bab, d_baa, d_baab, d_baba;
+    DevBuf<int> d_sym, d_gab, d_gaa, d_bab, d_baa, d_baab, d_baba, d_baaa;
     DevBuf<v2g::DevLayout> d_lay;
     DevBuf<v2g::Segment> d_rowsegs, d_xsegs, d_t2segs;
     DevBuf<int64_t> d_rowtiles, d_xtiles, d_t2tiles;
@@ -131,6 +131,7 @@ v2g_solver::v2g_solver(const v2g_layout_desc &d)
     o.q2 = d.constrain_q2;
     o.g2 = d.constrain_g2;
     o.t2 = d.constrain_t2;
+    o.t1 = d.constrain_t1;
     o.spin = d.constrain_spin;
     o.sz = d.constrain_sz;
     L.reset(new v2g::Layout(o));
@@ -144,6 +145,7 @@ v2g_solver::v2g_solver(const v2g_layout_desc &d)
     d_baa.upload(L->baa);
     d_baab.upload(L->baab);
     d_baba.upload(L->baba);
+    d_baaa.upload(L->baaa);
     v2g::DevLayout dl{};
     dl.nirrep = L->nirrep;
     dl.amo = L->amo;
@@ -153,6 +155,7 @@ v2g_solver::v2g_solver(const v2g_layout_desc &d)
     dl.q2 = o.q2;
     dl.g2 = o.g2;
     dl.t2 = o.t2;
+    dl.t1 = o.t1;
     dl.na = L->na;
     dl.nb = L->nb;
     for (int h = 0; h < 8; h++) {
@@ -162,6 +165,8 @@ v2g_solver::v2g_solver(const v2g_layout_desc &d)
         dl.gems_aa[h] = L->gems_aa[h];
         dl.trip_aab[h] = L->trip_aab[h];
         dl.trip_aba[h] = L->trip_aba[h];
+        dl.trip_aaa[h] = L->trip_aaa[h];
+        dl.baaa_off[h] = L->baaa_off[h];
         dl.bab_off[h] = L->bab_off[h];
         dl.baa_off[h] = L->baa_off[h];
         dl.baab_off[h] = L->baab_off[h];
@@ -176,6 +181,7 @@ v2g_solver::v2g_solver(const v2g_layout_desc &d)
     dl.baa = d_baa.p;
     dl.baab = d_baab.p;
     dl.baba = d_baba.p;
+    dl.baaa = d_baaa.p;
     d_lay.upload(std::vector<v2g::DevLayout>{dl});
 
     auto tiles = [](const std::vector<v2g::Segment> &segs, std::vector<int64_t> &ts) {

@@ -9,12 +9,13 @@
 namespace v2g {
 
 struct DevLayout {
-    int nirrep, amo, singlet, sz, spin, q2, g2, t2;
+    int nirrep, amo, singlet, sz, spin, q2, g2, t2, t1;
     double na, nb;
     int amopi[kMaxIrrep], poff[kMaxIrrep];
     int gems_ab[kMaxIrrep], gems_aa[kMaxIrrep];
-    int trip_aab[kMaxIrrep], trip_aba[kMaxIrrep];
+    int trip_aab[kMaxIrrep], trip_aba[kMaxIrrep], trip_aaa[kMaxIrrep];
     int64_t bab_off[kMaxIrrep], baa_off[kMaxIrrep], baab_off[kMaxIrrep], baba_off[kMaxIrrep];
+    int64_t baaa_off[kMaxIrrep];
     int64_t xoff[XB_COUNT][kMaxIrrep];
     int64_t rowoff[RF_COUNT][6][kMaxIrrep];
     // device tables
@@ -25,6 +26,7 @@ struct DevLayout {
     const int *baa;
     const int *baab;   // packed triplets
     const int *baba;
+    const int *baaa;   // i < j < k triplets (T1aaa / T1bbb)
 };
 
 }  // namespace v2g

@@ -9,8 +9,9 @@
 // column blocks.  Each round pairs the column blocks (circle method); a
 // workgroup diagonalizes each 64x64 pair sub-matrix in LDS by parallel
 // cyclic Jacobi, and a second kernel applies the pair rotations to A
-// (tile-wise Q_P^T A_PR Q_R) and to V.  Sweeps repeat until a full sweep
-// applies no rotation above tol = 1e-15 ||M||_F.
+// (tile-wise Q_P^T A_PR Q_R on FP64 MFMA) and to V.  Sweeps repeat until a
+// full sweep applies no rotation above tol = 2e-14 ||M||_F (blocks n <= 64:
+// 1e-15 ||M||_F).
 //
 // Warm start: the eigenvectors of the previous Update_xz stay on the device;
 // the next call starts from V^T M V, which is nearly diagonal once the BPSDP
@@ -268,7 +269,7 @@ k_pair_eig(const EigBlock *__restrict__ blocks, int round, const double *__restr
 // transX: X is read transposed.  Wave w owns the 32x32 output sub-tile
 // (rows 32 (w/2), cols 32 (w%2)); v_mfma_f64_16x16x4f64 operand layout:
 // A[i][k] from lane i + 16k, B[k][j] from lane j + 16k, D[i][j] in lane
-// j + 16 (i/4), element i%4.
+// j + 16 (i%4), element i/4 (one FP64 row per accumulator register).
 __device__ __forceinline__ void mfma_tile(const double *X, const double *Y, bool transX,
                                           v4d (&acc)[2][2])
 {
@@ -298,7 +299,7 @@ __device__ __forceinline__ void store_tile(double *T, const v4d (&acc)[2][2])
     for (int a = 0; a < 2; a++)
         for (int b = 0; b < 2; b++)
             for (int r = 0; r < 4; r++)
-                T[(wr + a * 16 + 4 * (lane >> 4) + r) * kLd + wc + b * 16 + (lane & 15)] = acc[a][b][r];
+                T[(wr + a * 16 + 4 * r + (lane >> 4)) * kLd + wc + b * 16 + (lane & 15)] = acc[a][b][r];
 }
 
 // A_PR <- Q_P^T A_PR Q_R for the pair tiles with P <= R (A is symmetric: the

@@ -2,8 +2,8 @@
 // (warm-start transform V^T M V and the reconstruction x = W W^T).
 // 64 x 64 output tile per workgroup, 16-deep K panels staged in LDS; each
 // wave computes a 32 x 32 sub-tile with v_mfma_f64_16x16x4f64 (A[i][k] from
-// lane i + 16k, B[k][j] from lane j + 16k, D[i][j] in lane j + 16 (i/4),
-// element i % 4).
+// lane i + 16k, B[k][j] from lane j + 16k, D[i][j] in lane j + 16 (i % 4),
+// element i / 4).
 #include <hip/hip_runtime.h>
 
 #include "eig.hpp"
@@ -68,7 +68,7 @@ k_gemm(const GemmDesc *__restrict__ descs)
     for (int a = 0; a < 2; a++)
         for (int b = 0; b < 2; b++)
             for (int r = 0; r < 4; r++) {
-                const int gm = m0 + wr + a * 16 + 4 * lk + r, gn = n0 + wc + b * 16 + li;
+                const int gm = m0 + wr + a * 16 + 4 * r + lk, gn = n0 + wc + b * 16 + li;
                 if (gm < d.M && gn < d.N) d.C[(int64_t)gm * d.ldc + gn] = acc[a][b][r];
             }
 }

@@ -56,13 +56,14 @@ void Layout::build_tables()
         gems_ab[h] = nab;
         gems_aa[h] = naa;
     }
-    if (opt.t2) {
-        if (amo > 255) throw std::invalid_argument("T2 needs amo < 256");
-        // triplets: i outer, j, k inner; aab keeps i < j (basis.cc:655-757)
+    if (opt.t2 || opt.t1) {
+        // triplets: i outer, j, k inner; aab keeps i < j, aaa i < j < k
+        // (basis.cc:655-757)
         for (int h = 0; h < nirrep; h++) {
             baab_off[h] = (int64_t)baab.size();
             baba_off[h] = (int64_t)baba.size();
-            int caab = 0, caba = 0;
+            baaa_off[h] = (int64_t)baaa.size();
+            int caab = 0, caba = 0, caaa = 0;
             for (int i = 0; i < amo; i++)
                 for (int j = 0; j < amo; j++)
                     for (int k = 0; k < amo; k++) {
@@ -73,15 +74,20 @@ void Layout::build_tables()
                         if (i >= j) continue;
                         baab.push_back(packed);
                         caab++;
+                        if (j >= k) continue;
+                        baaa.push_back(packed);
+                        caaa++;
                     }
             trip_aab[h] = caab;
             trip_aba[h] = caba;
+            trip_aaa[h] = caaa;
         }
     }
     if (bab.empty()) bab.push_back(0);
     if (baa.empty()) baa.push_back(0);
     if (baab.empty()) baab.push_back(0);
     if (baba.empty()) baba.push_back(0);
+    if (baaa.empty()) baaa.push_back(0);
 }
 
 void Layout::build_offsets()
@@ -119,6 +125,14 @@ void Layout::build_offsets()
         blocks(XB_G2AB, g);
         blocks(XB_G2BA, g);
         blocks(XB_G2AA, [&](int h) { return 2 * g(h); });
+    }
+    if (opt.t1) {  // v2rdm_solver.cc:809-825
+        auto t3 = [&](int h) { return (int64_t)trip_aaa[h]; };
+        auto t = [&](int h) { return (int64_t)trip_aab[h]; };
+        blocks(XB_T1AAA, t3);
+        blocks(XB_T1BBB, t3);
+        blocks(XB_T1AAB, t);
+        blocks(XB_T1BBA, t);
     }
     if (opt.t2) {
         auto T = [&](int h) { return (int64_t)trip_aab[h] + trip_aba[h]; };
@@ -201,6 +215,15 @@ void Layout::build_rows()
                 seg(RF_G2AB, s, h, gems_ab[h], (int64_t)gems_ab[h] * gems_ab[h], 0);
         for (int h = 0; h < nirrep; h++)
             seg(RF_G2AA, 0, h, gems_ab[h], 4LL * gems_ab[h] * gems_ab[h], 0);
+    }
+    // T1: aab, bba, aaa, bbb; t1.cc:433-809
+    if (opt.t1) {
+        for (int s = 0; s < 2; s++)
+            for (int h = 0; h < nirrep; h++)
+                seg(RF_T1AAB, s, h, trip_aab[h], (int64_t)trip_aab[h] * trip_aab[h], 0);
+        for (int s = 0; s < 2; s++)
+            for (int h = 0; h < nirrep; h++)
+                seg(RF_T1AAA, s, h, trip_aaa[h], (int64_t)trip_aaa[h] * trip_aaa[h], 0);
     }
     // T2 slow maps: aab, bba, aaa(+abb), bbb(+baa); t2.cc:1251-1861
     if (opt.t2) {
@@ -290,6 +313,22 @@ void Layout::build_xsegs()
                 rowoff[RF_G2AA][0][h], -1.0, 0);
         }
     }
+    if (opt.t1) {
+        // the triplet pass also walks the T1 rows: self term into the T1 block
+        // plus the delta-selected Q2 / D2 / G2 / D1 terms scattered atomically
+        for (int s = 0; s < 2; s++)
+            for (int h = 0; h < nirrep; h++) {
+                int64_t t = trip_aab[h];
+                add(xsegs_t2, RF_T1AAB, s, h, (int)t, xoff[s ? XB_T1BBA : XB_T1AAB][h], t * t,
+                    rowoff[RF_T1AAB][s][h], -1.0, 0);
+            }
+        for (int s = 0; s < 2; s++)
+            for (int h = 0; h < nirrep; h++) {
+                int64_t t = trip_aaa[h];
+                add(xsegs_t2, RF_T1AAA, s, h, (int)t, xoff[s ? XB_T1BBB : XB_T1AAA][h], t * t,
+                    rowoff[RF_T1AAA][s][h], -1.0, 0);
+            }
+    }
     if (opt.t2) {
         // the T2 pass walks the T2 rows: self term into the T2 block plus the
         // delta-selected D2 / D1 terms scattered atomically
@@ -312,7 +351,8 @@ std::vector<std::pair<std::string, std::pair<int64_t, int64_t>>> Layout::familie
 {
     static const char *names[RF_COUNT] = {
         "trace", "d1q1", "contract", "s2", "spin_d1", "spin_d2", "spin_anti", "spin_d200",
-        "spin_d200_4", "maxspin", "q2ab", "q2aa", "g2ab", "g2aa", "t2aab", "t2big"};
+        "spin_d200_4", "maxspin", "q2ab", "q2aa", "g2ab", "g2aa", "t2aab", "t2big",
+        "t1aab", "t1aaa"};
     std::vector<std::pair<std::string, std::pair<int64_t, int64_t>>> out;
     for (const auto &s : rowsegs) {
         std::string nm = names[s.kind];

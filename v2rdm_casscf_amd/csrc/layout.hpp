@@ -22,7 +22,8 @@ constexpr int kMaxIrrep = 8;
 enum XBlock : int {
     XB_D2AB = 0, XB_D2AA, XB_D2BB, XB_D200, XB_D1A, XB_D1B, XB_Q1A, XB_Q1B,
     XB_Q2AB, XB_Q2AA, XB_Q2BB, XB_G2AB, XB_G2BA, XB_G2AA,
-    XB_T2AAA, XB_T2BBB, XB_T2AAB, XB_T2BBA, XB_COUNT
+    XB_T2AAA, XB_T2BBB, XB_T2AAB, XB_T2BBA,
+    XB_T1AAA, XB_T1BBB, XB_T1AAB, XB_T1BBA, XB_COUNT
 };
 
 // constraint-row families in bpsdp_Au order
@@ -44,6 +45,8 @@ enum RowFam : int {
     RF_G2AA,           // 2g x 2g super block
     RF_T2AAB,          // sub: 0 T2aab, 1 T2bba
     RF_T2BIG,          // sub: 0 T2aaa(+abb), 1 T2bbb(+baa)
+    RF_T1AAB,          // sub: 0 T1aab, 1 T1bba
+    RF_T1AAA,          // sub: 0 T1aaa, 1 T1bbb
     RF_COUNT
 };
 
@@ -73,7 +76,7 @@ struct Options {
     int nirrep = 1;
     int amopi[kMaxIrrep] = {0};
     int nalpha = 0, nbeta = 0, multiplicity = 1;
-    bool q2 = true, g2 = true, t2 = false, spin = true, sz = true;
+    bool q2 = true, g2 = true, t2 = false, spin = true, sz = true, t1 = false;
 };
 
 class Layout {
@@ -89,9 +92,9 @@ public:
     int gems_ab[kMaxIrrep] = {0}, gems_aa[kMaxIrrep] = {0};
     std::vector<int> bab, baa;          // packed pairs p | q << 16, concatenated over irreps
     int64_t bab_off[kMaxIrrep] = {0}, baa_off[kMaxIrrep] = {0};
-    int trip_aab[kMaxIrrep] = {0}, trip_aba[kMaxIrrep] = {0};
-    std::vector<int> baab, baba;        // packed triplets i | j << 8 | k << 16
-    int64_t baab_off[kMaxIrrep] = {0}, baba_off[kMaxIrrep] = {0};
+    int trip_aab[kMaxIrrep] = {0}, trip_aba[kMaxIrrep] = {0}, trip_aaa[kMaxIrrep] = {0};
+    std::vector<int> baab, baba, baaa;  // packed triplets i | j << 8 | k << 16
+    int64_t baab_off[kMaxIrrep] = {0}, baba_off[kMaxIrrep] = {0}, baaa_off[kMaxIrrep] = {0};
 
     int64_t xoff[XB_COUNT][kMaxIrrep];  // -1 when the block is absent
     int64_t dimx = 0, ncon = 0;
@@ -102,7 +105,7 @@ public:
     int64_t rowoff[RF_COUNT][6][kMaxIrrep];
     std::vector<Segment> rowsegs;       // A.x segments, ascending start
     std::vector<Segment> xsegs;         // A^T.y segments (main pass)
-    std::vector<Segment> xsegs_t2;      // A^T.y T2 pass (self + scatter)
+    std::vector<Segment> xsegs_t2;      // A^T.y triplet pass (T1 / T2 self + scatter)
 
     std::vector<std::pair<std::string, std::pair<int64_t, int64_t>>> families() const;
 

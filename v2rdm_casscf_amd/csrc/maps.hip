@@ -220,7 +220,7 @@ k_atu_t2(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
         const double yv = y[sg.aux + e];
         ATy[sg.start + e] = -yv;
         Scatter f{ATy, yv};
-        t2_terms(L, sg.kind, sg.sub, sg.h, e, f, false);
+        triplet_terms(L, sg.kind, sg.sub, sg.h, e, f, false);
     }
 }
 

@@ -3,7 +3,8 @@
 // validates every index against the CPU oracle before a kernel runs).
 //
 // Row formulas follow d2.cc:296-547, spin.cc:274-493, q2.cc:801-950,
-// g2.cc:1061-1260 and t2.cc:1251-1861; the A^T gathers invert them.
+// g2.cc:1061-1260, t1.cc:433-809 and t2.cc:1251-1861; the A^T gathers
+// invert them.
 #pragma once
 #include <cmath>
 #include <cstdint>
@@ -118,6 +119,102 @@ V2G_HD inline void t2_terms(const DevLayout &L, int kind, int s, int h, int64_t 
         if (i == l) AA(w2o, n, j, k, m, -1.0);
         if (j == m) { if (s) AB(n, i, k, l, -1.0); else AB(i, n, l, k, -1.0); }
     }
+}
+
+// T1 rows, t1.cc:433-809.  kind RF_T1AAB: T1aab (s = 0) / T1bba (s = 1) over
+// i<j triplets; RF_T1AAA: T1aaa / T1bbb over i<j<k triplets.  At most one
+// term of each delta group fires, so a row never repeats a column.
+template <class F>
+V2G_HD inline void t1_terms(const DevLayout &L, int kind, int s, int h, int64_t e, F &f,
+                            bool self)
+{
+    const int amo = L.amo;
+    const int *gab = L.gab, *gaa = L.gaa, *sym = L.sym;
+    auto GAB = [&](int p, int q) -> int64_t { return gab[p * amo + q]; };
+    auto GAA = [&](int p, int q) -> int64_t { return gaa[p * amo + q]; };
+    const int wq = s ? XB_Q2BB : XB_Q2AA;
+    if (kind == RF_T1AAB) {
+        const int t = L.trip_aab[h];
+        const int *bt = L.baab + L.baab_off[h];
+        const int ijk = (int)(e / t), lmn = (int)(e % t);
+        const int a = bt[ijk], b = bt[lmn];
+        const int i = T0(a), j = T1(a), k = T2i(a), l = T0(b), m = T1(b), n = T2i(b);
+        if (self) f(L.xoff[s ? XB_T1BBA : XB_T1AAB][h] + e, -1.0);
+        if (k == n) {
+            const int hij = sym[i] ^ sym[j];
+            f(L.xoff[wq][hij] + GAA(i, j) * L.gems_aa[hij] + GAA(l, m), 1.0);
+        }
+        // D2ab (mn|ik): T1aab reads alpha-first pairs, T1bba the swapped ones
+        auto AB = [&](int p1, int q1, int p2, int q2, double c) {
+            const int hh = sym[p1] ^ sym[q1];
+            const int64_t a1 = s ? GAB(q1, p1) : GAB(p1, q1), a2 = s ? GAB(q2, p2) : GAB(p2, q2);
+            f(L.xoff[XB_D2AB][hh] + a1 * L.gems_ab[hh] + a2, c);
+        };
+        if (j == l) AB(m, n, i, k, -1.0);
+        if (l == i) AB(m, n, j, k, 1.0);
+        // mixed-spin G2: G2ba for T1aab, G2ab for T1bba
+        const int wg = s ? XB_G2AB : XB_G2BA;
+        if (j == m) {
+            const int hh = sym[n] ^ sym[i];
+            f(L.xoff[wg][hh] + GAB(n, i) * L.gems_ab[hh] + GAB(k, l), -1.0);
+        }
+        if (i == m) {
+            const int hh = sym[n] ^ sym[j];
+            f(L.xoff[wg][hh] + GAB(n, j) * L.gems_ab[hh] + GAB(k, l), 1.0);
+        }
+        return;
+    }
+    const int t = L.trip_aaa[h];
+    const int *bt = L.baaa + L.baaa_off[h];
+    const int ijk = (int)(e / t), lmn = (int)(e % t);
+    const int a = bt[ijk], b = bt[lmn];
+    const int i = T0(a), j = T1(a), k = T2i(a), l = T0(b), m = T1(b), n = T2i(b);
+    if (self) f(L.xoff[s ? XB_T1BBB : XB_T1AAA][h] + e, -1.0);
+    const int w2 = s ? XB_D2BB : XB_D2AA, wd = s ? XB_D1B : XB_D1A;
+    auto Q = [&](int p, int q, double c) {   // Q2 (pq|lm)
+        const int hh = sym[p] ^ sym[q];
+        f(L.xoff[wq][hh] + GAA(p, q) * L.gems_aa[hh] + GAA(l, m), c);
+    };
+    auto D = [&](int p, int q, double c) {   // D2 (nm|pq)
+        const int hh = sym[p] ^ sym[q];
+        f(L.xoff[w2][hh] + GAA(n, m) * L.gems_aa[hh] + GAA(p, q), c);
+    };
+    auto D1 = [&](int p, double c) {         // D1 (n, p)
+        const int h3 = sym[n];
+        f(L.xoff[wd][h3] + (int64_t)(n - L.poff[h3]) * L.amopi[h3] + (p - L.poff[h3]), c);
+    };
+    auto G = [&](int p, int q, int r, double c) {   // G2 super block (np|qr), spin quadrant s
+        const int hh = sym[n] ^ sym[p];
+        const int64_t g = L.gems_ab[hh], sh = s ? g : 0;
+        f(L.xoff[XB_G2AA][hh] + (GAB(n, p) + sh) * (2 * g) + (GAB(q, r) + sh), c);
+    };
+    if (k == n) Q(i, j, 1.0);
+    if (j == n) Q(i, k, -1.0);
+    if (i == n) Q(j, k, 1.0);
+    if (l == k) D(j, i, 1.0);
+    if (j == l) D(k, i, -1.0);
+    if (l == i) D(k, j, 1.0);
+    if (k == m) {
+        if (j == l) D1(i, -1.0);
+        G(i, j, l, 1.0);
+    }
+    if (j == m) {
+        if (k == l) D1(i, 1.0);
+        G(i, k, l, -1.0);
+    }
+    if (i == m) {
+        if (k == l) D1(j, -1.0);
+        G(j, k, l, 1.0);
+    }
+}
+
+// the rows walked by the triplet pass of A^T.y
+template <class F>
+V2G_HD inline void triplet_terms(const DevLayout &L, int kind, int s, int h, int64_t e, F &f,
+                                 bool self)
+{
+    if (kind == RF_T1AAB || kind == RF_T1AAA) t1_terms(L, kind, s, h, e, f, self);
+    else t2_terms(L, kind, s, h, e, f, self);
 }
 
 template <class F>
@@ -282,6 +379,10 @@ V2G_HD inline void elem_terms(const DevLayout &L, const Segment &sg, int64_t e, 
     case RF_T2AAB:
     case RF_T2BIG:
         t2_terms(L, sg.kind, s, h, e, f, true);
+        return;
+    case RF_T1AAB:
+    case RF_T1AAA:
+        t1_terms(L, sg.kind, s, h, e, f, true);
         return;
     default:
         return;

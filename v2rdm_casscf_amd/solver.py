@@ -25,8 +25,10 @@ import torch
 from . import _lib
 from ._lib import check
 
-POSITIVITY = {"D": (0, 0, 0), "DQ": (1, 0, 0), "DG": (0, 1, 0), "DQG": (1, 1, 0),
-              "DQGT2": (1, 1, 1)}
+# POSITIVITY -> (q2, g2, t1, t2), v2rdm_solver.cc:495-523
+POSITIVITY = {"D": (0, 0, 0, 0), "DQ": (1, 0, 0, 0), "DG": (0, 1, 0, 0), "DQG": (1, 1, 0, 0),
+              "DQGT1": (1, 1, 1, 0), "DQGT2": (1, 1, 0, 1), "DQGT1T2": (1, 1, 1, 1),
+              "DQGT": (1, 1, 1, 1)}
 
 
 def _ptr(t):
@@ -44,7 +46,8 @@ class V2RDMSolver:
                  constrain_spin=True, constrain_sz=True, device="cuda:0", options=None):
         pos = positivity.upper()
         if pos not in POSITIVITY:
-            raise ValueError(f"POSITIVITY {positivity} is not supported (D, DQ, DG, DQG, DQGT2)")
+            raise ValueError(f"POSITIVITY {positivity} is not supported "
+                             f"({', '.join(POSITIVITY)})")
         amopi = list(amopi)
         if len(amopi) not in (1, 2, 4, 8):
             raise ValueError("amopi must have 1, 2, 4 or 8 irreps")
@@ -56,7 +59,7 @@ class V2RDMSolver:
             d.amopi[h] = amopi[h] if h < len(amopi) else 0
         d.nalpha, d.nbeta = int(nalpha), int(nbeta)
         d.multiplicity = int(multiplicity) if multiplicity else int(nalpha - nbeta + 1)
-        d.constrain_q2, d.constrain_g2, d.constrain_t2 = POSITIVITY[pos]
+        d.constrain_q2, d.constrain_g2, d.constrain_t1, d.constrain_t2 = POSITIVITY[pos]
         d.constrain_spin, d.constrain_sz = int(constrain_spin), int(constrain_sz)
         self.desc = d
         self.amopi, self.nalpha, self.nbeta = amopi, nalpha, nbeta
