@@ -77,6 +77,37 @@ def cpu_baseline(w, gpu_cg_per_outer, seconds_budget=25.0):
                         f"Update_xz + ({gpu_cg_per_outer:.1f} CG + 4 map) applications"))
 
 
+def _maps_only(s, stream, reps, w):
+    """A^T.y and A.x alone (plus a reference copy) for hardware-counter runs."""
+    import ctypes
+    import torch
+    dev = s.y.device
+    ncon, dimx = s.nconstraints, s.dimx
+    yv = s.y.clone()
+    aty = torch.empty(dimx, dtype=torch.float64, device=dev)
+    ax = torch.empty(ncon, dtype=torch.float64, device=dev)
+    src = torch.empty(dimx + ncon, dtype=torch.float64, device=dev).fill_(1.0)
+    dst = torch.empty_like(src)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+    with torch.cuda.stream(stream):
+        ev[0].record(stream)
+        for _ in range(reps):
+            s.lib.v2g_ATu(s.h, ctypes.c_void_p(yv.data_ptr()), ctypes.c_void_p(aty.data_ptr()))
+        ev[1].record(stream)
+        for _ in range(reps):
+            s.lib.v2g_Au(s.h, ctypes.c_void_p(aty.data_ptr()), ctypes.c_void_p(ax.data_ptr()))
+        ev[2].record(stream)
+        for _ in range(reps):
+            dst.copy_(src)
+        ev[3].record(stream)
+    ev[3].synchronize()
+    out = dict(workload=w["name"], dimx=dimx, ncon=ncon, reps=reps,
+               bytes_map=8.0 * (dimx + ncon), bytes_copy=2 * 8.0 * (dimx + ncon),
+               t_atu_ms=ev[0].elapsed_time(ev[1]) / reps, t_au_ms=ev[1].elapsed_time(ev[2]) / reps,
+               t_copy_ms=ev[2].elapsed_time(ev[3]) / reps)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -88,6 +119,13 @@ def main():
     ap.add_argument("--map-reps", type=int, default=20)
     ap.add_argument("--eig-sweeps", type=int, default=3,
                     help="block-Jacobi sweeps per warm Update_xz inside the loop (0: converge)")
+    ap.add_argument("--positivity", default="DQG", help="D, DQ, DG, DQG, DQGT1, DQGT2, DQGT")
+    ap.add_argument("--converge", type=float, default=0.0,
+                    help="instead of timing K steps: run from the RANDOM guess to R/E_CONVERGENCE "
+                         "1e-6 (stopping after this many seconds) and report time-to-converge")
+    ap.add_argument("--maps-only", action="store_true",
+                    help="only the A^T.y / A.x kernels (map-reps calls each) and a device copy "
+                         "of 8 (dimx + ncon) bytes as the traffic reference, for counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box in this setup has a single MI355X; the "
@@ -99,7 +137,7 @@ def main():
 
     dev = torch.device("cuda:0")
     w = _workload(args.workload)
-    s = V2RDMSolver(w["amopi"], w["na"], w["nb"], 1, "DQG", device=dev,
+    s = V2RDMSolver(w["amopi"], w["na"], w["nb"], 1, args.positivity, device=dev,
                     options=dict(PRECONDITIONER=args.precond.upper(),
                                  EIG_WARM_SWEEPS=args.eig_sweeps))
     stream = torch.cuda.Stream(device=dev)
@@ -108,6 +146,24 @@ def main():
     s.BuildConstraints()
     s.Guess()
     torch.cuda.synchronize()
+    if args.maps_only:
+        _maps_only(s, stream, args.map_reps, w)
+        return
+    if args.converge > 0:
+        s.options["MAXITER"] = 10 ** 7
+        t0 = time.perf_counter()
+        st = s.compute_energy(guess=False, max_seconds=args.converge)
+        torch.cuda.synchronize()
+        secs = time.perf_counter() - t0
+        keep = ("oiter", "cg_iters", "converged", "ep", "ed", "mu", "t_cg", "t_xz",
+                "precond_active", "fallback", "cg_iters_train", "train_outer", "mlp_loss")
+        print(json.dumps(dict(mode="time-to-converge", workload=w["name"],
+                              positivity=args.positivity, precond=args.precond,
+                              dimx=s.dimx, ncon=s.nconstraints, seconds=round(secs, 3),
+                              energy=st["energy_primal"] + w["enuc"],
+                              energy_dual=st["energy_dual"] + w["enuc"],
+                              **{k: st[k] for k in keep if k in st})))
+        return
 
     # warmup outer iterations (includes the cold eigen sweeps of the guess)
     s.options["MAXITER"] = args.warmup
