@@ -110,6 +110,11 @@ typedef struct v2g_params {
     double train_lr;             /* precond 3: Adam learning rate         */
     int eig_warm_sweeps;         /* > 0: block-Jacobi sweeps per warm      */
                                  /* Update_xz inside the loop (0: converge) */
+    int eig_inner_sweeps;        /* > 0: cyclic-Jacobi sweeps per 64 x 64  */
+                                 /* pair sub-problem of a warm call (0: 12) */
+    int cg_extrapolate;          /* 1: start each CG from 2 y_k - y_(k-1)  */
+                                 /* (linear extrapolation of the last two */
+                                 /* solutions); 0: from y_k (reference)   */
 } v2g_params;
 
 typedef struct v2g_stats {

@@ -28,7 +28,8 @@ class Params(ctypes.Structure):
                 ("flexible", ctypes.c_int), ("verbose", ctypes.c_int),
                 ("max_seconds", ctypes.c_double), ("train_iters", ctypes.c_int),
                 ("train_batch", ctypes.c_int), ("train_steps", ctypes.c_int),
-                ("train_lr", ctypes.c_double), ("eig_warm_sweeps", ctypes.c_int)]
+                ("train_lr", ctypes.c_double), ("eig_warm_sweeps", ctypes.c_int),
+                ("eig_inner_sweeps", ctypes.c_int), ("cg_extrapolate", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):

@@ -79,6 +79,10 @@ struct v2g_solver {
     bool have_jacobi = false, have_user = false;
     // loop scratch
     DevBuf<double> Ax, Bv, ATy, red;
+    DevBuf<double> yhist;   // previous outer iteration's CG solution
+    int mlp_outer_trained = 0;  // outer iterations of MLP training done so far
+    bool mlp_fallback = false, mlp_first_learned = true;
+    int mlp_last_base_iters = 0;   // CG iterations of the last training solve
     // eigen split
     std::vector<v2g::EigBlock> small, large;
     DevBuf<v2g::EigBlock> d_small, d_large;
@@ -116,7 +120,7 @@ struct v2g_solver {
     int cg(const double *rhs, double *y, double tol, int maxiter, int precond, int flexible,
            double *res, double *r0_out = nullptr);
     void update_xz(double mu, const double *y, const double *c, double *x, double *z,
-                   int warm_sweep_cap = 0);
+                   int warm_sweep_cap = 0, int inner_sweep_cap = 0);
     double dot(const double *a, const double *b, int64_t n);
 };
 
@@ -319,6 +323,10 @@ void v2g_solver::mlp_init(unsigned long long seed)
     HIPCHK(hipMemsetAsync(av.p, 0, sizeof(double) * v2g::kMlpParams, st));
     HIPCHK(hipStreamSynchronize(st));
     adam_t = 0;
+    mlp_outer_trained = 0;
+    mlp_fallback = false;
+    mlp_first_learned = true;
+    mlp_last_base_iters = 0;
 }
 
 // scales[0] = 1/rms(rhs), scales[1] = 1/rms(y)
@@ -441,7 +449,7 @@ int v2g_solver::cg(const double *rhs, double *y, double tolv, int maxiter, int p
 // becomes exact at the fixed point while early calls stay cheap.  x is always
 // rebuilt PSD from the current V; z = mu x - M keeps M exact.
 void v2g_solver::update_xz(double mu, const double *y, const double *c, double *x, double *zv,
-                           int warm_sweep_cap)
+                           int warm_sweep_cap, int inner_sweep_cap)
 {
     v2g::launch_atu(plan, y, ATy.p, st);
     v2g::eig_small(d_small.p, (int)small.size(), ATy.p, c, x, zv, mu, st);
@@ -468,12 +476,13 @@ void v2g_solver::update_xz(double mu, const double *y, const double *c, double *
         v2g::gemm_batched(d_gemm_warm2.p, nl, max_N, max_N, st);
     }
     const int maxsweeps = (!cold && warm_sweep_cap > 0) ? warm_sweep_cap : 40;
+    const int inner = (!cold && inner_sweep_cap > 0) ? inner_sweep_cap : 12;
     int sweeps = 0;
     for (; sweeps < maxsweeps; sweeps++) {
         HIPCHK(hipMemsetAsync(counter.p, 0, sizeof(int), st));
         for (int rnd = 0; rnd < max_nb - 1; rnd++)
             v2g::eig_round(d_large.p, nl, max_pairs, max_tiles, rnd, Aw.p, Vw.p, Qbuf.p, active.p,
-                           tol.p, counter.p, st);
+                           tol.p, counter.p, inner, st);
         int cnt = 0;
         HIPCHK(hipMemcpyAsync(&cnt, counter.p, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
@@ -661,8 +670,9 @@ int v2g_bpsdp(v2g_handle h, const v2g_params *prm, const double *c, const double
     std::mt19937_64 rng(20240607ULL);
     long long cg_train = 0;
     int n_train = 0;
-    bool fallback = false, first_learned = true;
-    int last_jacobi_iters = 0;
+    // the A/B guard state lives in the handle, so a later call keeps the verdict
+    bool &fallback = h->mlp_fallback, &first_learned = h->mlp_first_learned;
+    int &last_jacobi_iters = h->mlp_last_base_iters;
     double last_loss = 0.0;
     do {
         const double t1 = now();
@@ -678,7 +688,9 @@ int v2g_bpsdp(v2g_handle h, const v2g_params *prm, const double *c, const double
         if (cgtol < prm->cg_convergence) cgtol = prm->cg_convergence;
         double res = 0.0;
         int pc = prm->precond, flex = prm->flexible;
-        const bool training = learned && oiter < prm->train_iters;
+        // training spans the first train_iters outer iterations of the
+        // handle's life (it carries over between calls, like the weights)
+        const bool training = learned && h->mlp_outer_trained < prm->train_iters;
         if (learned) {
             if (training || fallback) {
                 pc = 1;
@@ -688,6 +700,17 @@ int v2g_bpsdp(v2g_handle h, const v2g_params *prm, const double *c, const double
                 pc = 4;
                 flex = 1;
             }
+        }
+        // CG starting guess: the reference restarts from y_k; with
+        // cg_extrapolate the guess is 2 y_k - y_(k-1), which cuts the CG work
+        // without changing the system, the tolerance or the outer iteration
+        if (prm->cg_extrapolate) {
+            if (!h->yhist.p) h->yhist.alloc(ncon);
+            if (oiter == 0)
+                HIPCHK(hipMemcpyAsync(h->yhist.p, y, sizeof(double) * ncon, hipMemcpyDeviceToDevice,
+                                      st));
+            else
+                v2g::vec_extrapolate(ncon, y, h->yhist.p, st);
         }
         if (training)
             HIPCHK(hipMemcpyAsync(h->yprev.p, y, sizeof(double) * ncon, hipMemcpyDeviceToDevice, st));
@@ -703,6 +726,7 @@ int v2g_bpsdp(v2g_handle h, const v2g_params *prm, const double *c, const double
                                      prm->train_lr > 0 ? prm->train_lr : 2e-3, rng);
             cg_train += iiter;
             n_train++;
+            h->mlp_outer_trained++;
             last_jacobi_iters = iiter;
         } else if (pc == 4) {
             // the true residual is the judge: abandon the learned diagonal if it
@@ -715,7 +739,7 @@ int v2g_bpsdp(v2g_handle h, const v2g_params *prm, const double *c, const double
         s.precond_active = pc == 4 ? 3 : pc;
         const double t2 = now();
         tcg += t2 - t1;
-        h->update_xz(mu, y, c, x, z, prm->eig_warm_sweeps);
+        h->update_xz(mu, y, c, x, z, prm->eig_warm_sweeps, prm->eig_inner_sweeps);
         HIPCHK(hipStreamSynchronize(st));
         txz += now() - t2;
         // ed = ||A^T y - c + z||, ep = ||A x - b||  (v2rdm_solver.cc:1829-1837)

@@ -226,6 +226,18 @@ k_mul(int64_t n, const double *a, const double *b, double *out)
         out[i] = a[i] * b[i];
 }
 
+// CG starting guess from the last two solutions: y <- 2 y - yh, yh <- old y
+__global__ void __launch_bounds__(kThreads)
+k_extrapolate(int64_t n, double *__restrict__ y, double *__restrict__ yh)
+{
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kThreads) {
+        const double yc = y[i];
+        y[i] = 2.0 * yc - yh[i];
+        yh[i] = yc;
+    }
+}
+
 __global__ void k_reduce(const double *__restrict__ partial, double *__restrict__ out)
 {
     __shared__ double sh[kThreads / 64];
@@ -288,6 +300,11 @@ void vec_dot_partial(int64_t n, const double *a, const double *b, double *partia
 void vec_mul(int64_t n, const double *a, const double *b, double *out, hipStream_t st)
 {
     hipLaunchKernelGGL(k_mul, dim3(kRedBlocks), dim3(kThreads), 0, st, n, a, b, out);
+}
+
+void vec_extrapolate(int64_t n, double *y, double *yh, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_extrapolate, dim3(kRedBlocks), dim3(kThreads), 0, st, n, y, yh);
 }
 
 void vec_reduce(const double *partial, double *out, hipStream_t st)

@@ -108,6 +108,133 @@ __device__ int lds_jacobi(double *S, double *Q, double tol, int maxsweeps, int *
     return total;
 }
 
+// The pair sub-problems of the large-block path use a leaner variant:
+//  * S keeps only its upper triangle (element (i, j) lives at (min, max)); a
+//    step updates the 528 pair blocks (P <= R) instead of all 1024;
+//  * Q lives in registers: wave w, lane l holds column l, rows 16w .. 16w+15.
+//    A column rotation needs the partner column only, which sits in the
+//    partner lane of the same wave, so Q moves through lane shuffles and never
+//    touches LDS.  That leaves one 64 x 65 FP64 array of LDS per workgroup.
+// In step r of the circle method, lane l is paired with 63 <-> r and
+// l <-> (2r - l) mod 63 otherwise; kk(r, l) is the pair's slot.
+__device__ __forceinline__ int circle_partner(int r, int l)
+{
+    if (l == kP - 1) return r;
+    if (l == r) return kP - 1;
+    return (2 * r - l + 2 * (kP - 1)) % (kP - 1);
+}
+
+__device__ __forceinline__ int circle_slot(int r, int l)
+{
+    if (l == kP - 1 || l == r) return 0;
+    const int d = (l - r + (kP - 1)) % (kP - 1);
+    return d <= kP / 2 - 1 ? d : (r - l + (kP - 1)) % (kP - 1);
+}
+
+constexpr int kUT = (kP / 2) * (kP / 2 + 1) / 2;   // 528 pair blocks with P <= R
+
+__device__ int pair_jacobi(double *S, double (&q)[kP / 4], double tol, int maxsweeps,
+                           const unsigned short *ut, int *sh_cnt, double *sh_c, double *sh_s,
+                           int *sh_pa, int *sh_pb)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    int total = 0;
+    for (int sweep = 0; sweep < maxsweeps; sweep++) {
+        if (threadIdx.x == 0) *sh_cnt = 0;
+        __syncthreads();
+        for (int r = 0; r < kP - 1; r++) {
+            const int pl = circle_partner(r, lane);
+            if (wave == 0) {
+                const int p = lane < pl ? lane : pl, qq = lane < pl ? pl : lane;
+                bool rotated = false;
+                if (lane == p) {
+                    const double apq = S[p * kLd + qq];
+                    double c = 1.0, sn = 0.0;
+                    if (fabs(apq) > tol) {
+                        const double app = S[p * kLd + p], aqq = S[qq * kLd + qq];
+                        const double th = (aqq - app) / (2.0 * apq);
+                        const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                        c = 1.0 / sqrt(t * t + 1.0);
+                        sn = t * c;
+                        rotated = true;
+                    }
+                    const int k = circle_slot(r, lane);
+                    sh_c[k] = c;
+                    sh_s[k] = sn;
+                    sh_pa[k] = p;
+                    sh_pb[k] = qq;
+                }
+                const unsigned long long b = __ballot(rotated);
+                if (lane == 0 && b) *sh_cnt += __popcll(b);
+            }
+            __syncthreads();
+            // S <- R^T S R on the upper-triangle pair blocks.  A thread owns up
+            // to kUTper blocks; all their LDS reads are issued before any
+            // write (the blocks are disjoint), so their latencies overlap.
+            {
+                constexpr int kUTper = (kUT + kT - 1) / kT;
+                int addr[kUTper][4];
+                double bv[kUTper][4], cf[kUTper][4];
+                bool act[kUTper];
+#pragma unroll
+                for (int u = 0; u < kUTper; u++) {
+                    const int w = threadIdx.x + u * kT;
+                    act[u] = false;
+                    if (w < kUT) {
+                        const int P = ut[w] & 0xff, R = ut[w] >> 8;
+                        cf[u][0] = sh_c[P];
+                        cf[u][1] = sh_s[P];
+                        cf[u][2] = sh_c[R];
+                        cf[u][3] = sh_s[R];
+                        act[u] = !(cf[u][1] == 0.0 && cf[u][3] == 0.0);
+                        const int p0 = sh_pa[P], p1 = sh_pb[P], r0 = sh_pa[R], r1 = sh_pb[R];
+                        auto ad = [](int i, int j) { return i < j ? i * kLd + j : j * kLd + i; };
+                        addr[u][0] = ad(p0, r0);
+                        addr[u][1] = ad(p0, r1);
+                        addr[u][2] = ad(p1, r0);
+                        addr[u][3] = ad(p1, r1);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < kUTper; u++)
+                    if (act[u]) {
+#pragma unroll
+                        for (int e = 0; e < 4; e++) bv[u][e] = S[addr[u][e]];
+                    }
+#pragma unroll
+                for (int u = 0; u < kUTper; u++)
+                    if (act[u]) {
+                        const double cp = cf[u][0], sp = cf[u][1], cr = cf[u][2], sr = cf[u][3];
+                        const double b00 = bv[u][0], b01 = bv[u][1], b10 = bv[u][2], b11 = bv[u][3];
+                        const double u00 = cp * b00 - sp * b10, u01 = cp * b01 - sp * b11;
+                        const double u10 = sp * b00 + cp * b10, u11 = sp * b01 + cp * b11;
+                        S[addr[u][0]] = cr * u00 - sr * u01;
+                        S[addr[u][1]] = sr * u00 + cr * u01;
+                        S[addr[u][2]] = cr * u10 - sr * u11;
+                        S[addr[u][3]] = sr * u10 + cr * u11;
+                    }
+            }
+            // Q <- Q R in registers; every lane takes part in the shuffles
+            {
+                const int k = circle_slot(r, lane);
+                const double c = sh_c[k], sn = sh_s[k];
+                const bool first = lane < pl;
+#pragma unroll
+                for (int i = 0; i < kP / 4; i++) {
+                    const double o = __shfl(q[i], pl, 64);
+                    q[i] = first ? c * q[i] - sn * o : sn * o + c * q[i];
+                }
+            }
+            __syncthreads();
+        }
+        const int cnt = *sh_cnt;
+        __syncthreads();
+        total += cnt;
+        if (cnt == 0) break;
+    }
+    return total;
+}
+
 // ---- small blocks: whole problem in one workgroup --------------------------
 __global__ void __launch_bounds__(kT)
 k_small(const EigBlock *__restrict__ blocks, const double *__restrict__ aty,
@@ -224,12 +351,12 @@ __device__ __forceinline__ int pair_index(int a, int b, int i)
 __global__ void __launch_bounds__(kT)
 k_pair_eig(const EigBlock *__restrict__ blocks, int round, const double *__restrict__ Aw,
            double *__restrict__ Qbuf, int *__restrict__ active, const double *__restrict__ tol,
-           int *__restrict__ counter)
+           int *__restrict__ counter, int inner_sweeps)
 {
     __shared__ double S[kP * kLd];
-    __shared__ double Q[kP * kLd];
     __shared__ double sh_c[kP / 2], sh_s[kP / 2];
     __shared__ int sh_pa[kP / 2], sh_pb[kP / 2], sh_cnt;
+    __shared__ unsigned short ut[kUT];
     const EigBlock B = blocks[blockIdx.y];
     const int nbk = B.N / 32;
     const int k = blockIdx.x;
@@ -239,13 +366,17 @@ k_pair_eig(const EigBlock *__restrict__ blocks, int round, const double *__restr
     const double *A = Aw + B.woff;
     const double tl = tol[blockIdx.y];
     if (threadIdx.x == 0) sh_cnt = 0;
+    for (int w = threadIdx.x; w < kUT; w += kT) {
+        int P = 0, rem = w;
+        while (rem >= kP / 2 - P) { rem -= kP / 2 - P; P++; }
+        ut[w] = (unsigned short)(P | ((P + rem) << 8));
+    }
     __syncthreads();
     int big = 0;
     for (int w = threadIdx.x; w < kP * kP; w += kT) {
         const int i = w / kP, j = w % kP;
         const double v = A[(int64_t)pair_index(a, b, i) * B.N + pair_index(a, b, j)];
         S[i * kLd + j] = v;
-        Q[i * kLd + j] = (i == j) ? 1.0 : 0.0;
         big |= (i != j && fabs(v) > tl);
     }
     if (big) atomicOr(&sh_cnt, 1);
@@ -256,9 +387,14 @@ k_pair_eig(const EigBlock *__restrict__ blocks, int round, const double *__restr
         if (threadIdx.x == 0) active[B.poff + k] = 0;
         return;
     }
-    const int rot = lds_jacobi(S, Q, tl, 12, &sh_cnt, sh_c, sh_s, sh_pa, sh_pb);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    double q[kP / 4];
+#pragma unroll
+    for (int i = 0; i < kP / 4; i++) q[i] = (wave * (kP / 4) + i == lane) ? 1.0 : 0.0;
+    const int rot = pair_jacobi(S, q, tl, inner_sweeps, ut, &sh_cnt, sh_c, sh_s, sh_pa, sh_pb);
     double *Qo = Qbuf + B.qoff + (int64_t)k * kP * kP;
-    for (int w = threadIdx.x; w < kP * kP; w += kT) Qo[w] = Q[(w / kP) * kLd + (w % kP)];
+#pragma unroll
+    for (int i = 0; i < kP / 4; i++) Qo[(wave * (kP / 4) + i) * kP + lane] = q[i];
     if (threadIdx.x == 0) {
         active[B.poff + k] = rot > 0;
         if (rot > 0) atomicAdd(counter, 1);
@@ -425,10 +561,10 @@ void eig_tol(const EigBlock *blocks, int nblocks, const double *Mw, double *tol,
 
 void eig_round(const EigBlock *blocks, int nblocks, int maxpairs, int64_t maxtiles, int round,
                double *Aw, double *Vw, double *Qbuf, int *active, const double *tol, int *counter,
-               hipStream_t st)
+               int inner_sweeps, hipStream_t st)
 {
     hipLaunchKernelGGL(k_pair_eig, dim3(maxpairs, nblocks), dim3(kT), 0, st, blocks, round, Aw, Qbuf,
-                       active, tol, counter);
+                       active, tol, counter, inner_sweeps);
     hipLaunchKernelGGL(k_apply, dim3((unsigned)maxtiles, nblocks), dim3(kT), 0, st, blocks, round, Aw,
                        Vw, Qbuf, active);
 }

@@ -33,9 +33,10 @@ void eig_build_m(const EigBlock *blocks, int nblocks, const double *aty, const d
 // tol[b] = 2e-14 ||M_b||_F; part holds 64 * nblocks doubles
 void eig_tol(const EigBlock *blocks, int nblocks, const double *Mw, double *tol, double *part,
              hipStream_t st);
+// inner_sweeps: cap on the cyclic-Jacobi sweeps of each 64 x 64 pair sub-problem
 void eig_round(const EigBlock *blocks, int nblocks, int maxpairs, int64_t maxtiles, int round,
                double *Aw, double *Vw, double *Qbuf, int *active, const double *tol, int *counter,
-               hipStream_t st);
+               int inner_sweeps, hipStream_t st);
 void eig_scale(const EigBlock *blocks, int nblocks, const double *Aw, const double *Vw, double *Tw,
                double mu, hipStream_t st);
 void eig_z(const EigBlock *blocks, int nblocks, const double *x, const double *Mw, double *z,

@@ -42,7 +42,7 @@ enum CgSlot : int {
     CG_RZ0 = 16, CG_RZ1 = 17, CG_DONE0 = 18, CG_DONE1 = 19, CG_NRM = 20, CG_ITER = 21,
     CG_ALPHA = 22, CG_BETA = 23, CG_PAP = 24, CG_TOL = 25, CG_MAXIT = 26, CG_NSLOTS = 32
 };
-constexpr int kRedBlocks = 512;     // partial-sum slots per reduction
+constexpr int kRedBlocks = 2048;    // partial-sum slots per reduction (= grid of the vector kernels)
 constexpr int kPartialArrays = 4;   // partial buffer holds 4 * kRedBlocks doubles
 
 void cg_init(int64_t n, const double *b, const double *Ap, double *r, double *p, const double *minv,
@@ -62,6 +62,8 @@ void vec_axpby(int64_t n, double a, const double *x, double b, const double *y, 
                hipStream_t st);
 void vec_dot_partial(int64_t n, const double *a, const double *b, double *partial, hipStream_t st);
 void vec_reduce(const double *partial, double *out, hipStream_t st);
+// y <- 2 y - yh and yh <- old y (linear extrapolation of the CG starting guess)
+void vec_extrapolate(int64_t n, double *y, double *yh, hipStream_t st);
 void vec_mul(int64_t n, const double *a, const double *b, double *out, hipStream_t st);
 
 }  // namespace v2g

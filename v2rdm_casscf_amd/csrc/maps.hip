@@ -57,6 +57,18 @@ struct Scatter {
     }
 };
 
+// Workgroups are dealt to the 8 XCDs round-robin (workgroup i runs on XCD
+// i % 8).  xcd_tile gives each XCD one contiguous eighth of the tiles, so the
+// rows (or primal entries) that one L2 serves are neighbours and their
+// permuted gathers land in a compact part of the input.
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ int64_t xcd_tile(int64_t i, int64_t nb)
+{
+    const int64_t x = i % kXcds, q = nb / kXcds, rem = nb % kXcds;
+    return x * q + (x < rem ? x : rem) + i / kXcds;
+}
+
 __device__ __forceinline__ int find_segment(const int64_t *__restrict__ tile_start, int nseg,
                                             int64_t b)
 {
@@ -78,9 +90,10 @@ k_au(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
     if (done && *done != 0.0) return;
     __shared__ double red[kWavesPerBlock];
     const DevLayout &L = *Lp;
-    const int si = find_segment(tile_start, nseg, blockIdx.x);
+    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int si = find_segment(tile_start, nseg, tile);
     const Segment sg = segs[si];
-    const int64_t t = blockIdx.x - tile_start[si];
+    const int64_t t = tile - tile_start[si];
     double contrib = 0.0;
     if (sg.wave) {
         const int64_t item = t * kWavesPerBlock + (threadIdx.x >> 6);
@@ -121,7 +134,7 @@ k_au(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
         if (threadIdx.x == 0) {
             double s2 = 0.0;
             for (int w = 0; w < kWavesPerBlock; w++) s2 += red[w];
-            rowpart[blockIdx.x] = s2;
+            rowpart[tile] = s2;
         }
     }
 }
@@ -134,9 +147,10 @@ k_row_norm2(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
             const int64_t *__restrict__ tile_start, int nseg, double *__restrict__ d)
 {
     const DevLayout &L = *Lp;
-    const int si = find_segment(tile_start, nseg, blockIdx.x);
+    const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int si = find_segment(tile_start, nseg, tile);
     const Segment sg = segs[si];
-    const int64_t t = blockIdx.x - tile_start[si];
+    const int64_t t = tile - tile_start[si];
     if (sg.wave) {
         const int64_t item = t * kWavesPerBlock + (threadIdx.x >> 6);
         if (item >= sg.count) return;
@@ -173,9 +187,13 @@ k_atu(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
 {
     if (done && *done != 0.0) return;
     const DevLayout &L = *Lp;
-    const int si = find_segment(tile_start, nseg, blockIdx.x);
+    // blockIdx order (not xcd_tile): the primal segments differ widely in cost
+    // (closed-form D2 gathers vs copies), and contiguous per-XCD ranges would
+    // leave some XCDs with all of the expensive tiles
+    const int64_t tile = blockIdx.x;
+    const int si = find_segment(tile_start, nseg, tile);
     const Segment sg = segs[si];
-    const int64_t t = blockIdx.x - tile_start[si];
+    const int64_t t = tile - tile_start[si];
     if (sg.wave) {
         const int64_t item = t * kWavesPerBlock + (threadIdx.x >> 6);
         if (item >= sg.count) return;
@@ -212,10 +230,11 @@ k_atu_t2(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
 {
     if (done && *done != 0.0) return;
     const DevLayout &L = *Lp;
-    const int si = find_segment(tile_start, nseg, blockIdx.x);
+    const int64_t tile = blockIdx.x;
+    const int si = find_segment(tile_start, nseg, tile);
     const Segment sg = segs[si];
     for (int u = 0; u < kPerThread; u++) {
-        const int64_t e = (blockIdx.x - tile_start[si]) * kElemTile + u * kBlock + threadIdx.x;
+        const int64_t e = (tile - tile_start[si]) * kElemTile + u * kBlock + threadIdx.x;
         if (e >= sg.count) return;
         const double yv = y[sg.aux + e];
         ATy[sg.start + e] = -yv;

@@ -27,6 +27,22 @@ V2G_HD inline int T0(int packed) { return packed & 0xff; }
 V2G_HD inline int T1(int packed) { return (packed >> 8) & 0xff; }
 V2G_HD inline int T2i(int packed) { return (packed >> 16) & 0xff; }
 
+// e = q d + r for an element index inside one block: 32-bit unsigned
+// division whenever d^2 fits (every block below 65536 x 65536), 64-bit
+// otherwise.  The test is uniform across a workgroup.
+V2G_HD inline void split_index(int64_t e, int64_t d, int &q, int &r)
+{
+    if (d <= 65535) {
+        const uint32_t ue = (uint32_t)e, ud = (uint32_t)d;
+        const uint32_t qq = ue / ud;
+        q = (int)qq;
+        r = (int)(ue - qq * ud);
+    } else {
+        q = (int)(e / d);
+        r = (int)(e - (int64_t)q * d);
+    }
+}
+
 // ----------------------------------------------------------------------------
 // row terms of the elementwise families (A.x)
 // ----------------------------------------------------------------------------
@@ -48,7 +64,8 @@ V2G_HD inline void t2_terms(const DevLayout &L, int kind, int s, int h, int64_t 
         // T2aab / T2bba rows, t2.cc:1256-1390
         const int t = L.trip_aab[h];
         const int *bt = L.baab + L.baab_off[h];
-        const int ijk = (int)(e / t), lmn = (int)(e % t);
+        int ijk, lmn;
+        split_index(e, t, ijk, lmn);
         const int a = bt[ijk], b = bt[lmn];
         const int i = T0(a), j = T1(a), k = T2i(a), l = T0(b), m = T1(b), n = T2i(b);
         if (self) f(L.xoff[s ? XB_T2BBA : XB_T2AAB][h] + e, -1.0);
@@ -73,7 +90,8 @@ V2G_HD inline void t2_terms(const DevLayout &L, int kind, int s, int h, int64_t 
     // big blocks T2aaa(+abb) / T2bbb(+baa), t2.cc:1392-1858
     const int ta = L.trip_aab[h], tb = L.trip_aba[h];
     const int64_t T = (int64_t)ta + tb;
-    const int R = (int)(e / T), C = (int)(e % T);
+    int R, C;
+        split_index(e, T, R, C);
     const bool rA = R < ta, cA = C < ta;
     const int pr = rA ? L.baab[L.baab_off[h] + R] : L.baba[L.baba_off[h] + (R - ta)];
     const int pc = cA ? L.baab[L.baab_off[h] + C] : L.baba[L.baba_off[h] + (C - ta)];
@@ -136,7 +154,8 @@ V2G_HD inline void t1_terms(const DevLayout &L, int kind, int s, int h, int64_t 
     if (kind == RF_T1AAB) {
         const int t = L.trip_aab[h];
         const int *bt = L.baab + L.baab_off[h];
-        const int ijk = (int)(e / t), lmn = (int)(e % t);
+        int ijk, lmn;
+        split_index(e, t, ijk, lmn);
         const int a = bt[ijk], b = bt[lmn];
         const int i = T0(a), j = T1(a), k = T2i(a), l = T0(b), m = T1(b), n = T2i(b);
         if (self) f(L.xoff[s ? XB_T1BBA : XB_T1AAB][h] + e, -1.0);
@@ -166,7 +185,8 @@ V2G_HD inline void t1_terms(const DevLayout &L, int kind, int s, int h, int64_t 
     }
     const int t = L.trip_aaa[h];
     const int *bt = L.baaa + L.baaa_off[h];
-    const int ijk = (int)(e / t), lmn = (int)(e % t);
+    int ijk, lmn;
+        split_index(e, t, ijk, lmn);
     const int a = bt[ijk], b = bt[lmn];
     const int i = T0(a), j = T1(a), k = T2i(a), l = T0(b), m = T1(b), n = T2i(b);
     if (self) f(L.xoff[s ? XB_T1BBB : XB_T1AAA][h] + e, -1.0);
@@ -226,7 +246,9 @@ V2G_HD inline void elem_terms(const DevLayout &L, const Segment &sg, int64_t e, 
     auto GAA = [&](int p, int q) -> int64_t { return gaa[p * amo + q]; };
     switch (sg.kind) {
     case RF_D1Q1: {  // d2.cc:370-388
-        const int n = sg.dim, i = (int)(e / n), j = (int)(e % n);
+        const int n = sg.dim;
+        int i, j;
+        split_index(e, n, i, j);
         f(L.xoff[s ? XB_D1B : XB_D1A][h] + j * n + i, 1.0);
         f(L.xoff[s ? XB_Q1B : XB_Q1A][h] + e, 1.0);
         return;
@@ -241,7 +263,8 @@ V2G_HD inline void elem_terms(const DevLayout &L, const Segment &sg, int64_t e, 
         return;
     case RF_SPIN_ANTI: {  // spin.cc:336-373
         const int ga = sg.dim;
-        const int IJ = (int)(e / ga), KL = (int)(e % ga);
+        int IJ, KL;
+        split_index(e, ga, IJ, KL);
         const int pij = L.baa[L.baa_off[h] + IJ], pkl = L.baa[L.baa_off[h] + KL];
         const int i = P0(pij), j = P1(pij), k = P0(pkl), l = P1(pkl);
         const int64_t g = L.gems_ab[h], b = L.xoff[XB_D2AB][h];
@@ -255,7 +278,8 @@ V2G_HD inline void elem_terms(const DevLayout &L, const Segment &sg, int64_t e, 
     }
     case RF_SPIN_D200: {  // spin.cc:375-395
         const int g = sg.dim;
-        const int IJ = (int)(e / g), KL = (int)(e % g);
+        int IJ, KL;
+        split_index(e, g, IJ, KL);
         const int pij = L.bab[L.bab_off[h] + IJ], pkl = L.bab[L.bab_off[h] + KL];
         const int i = P0(pij), j = P1(pij), k = P0(pkl), l = P1(pkl);
         const int64_t ji = GAB(j, i), lk = GAB(l, k), b = L.xoff[XB_D2AB][h];
@@ -272,7 +296,8 @@ V2G_HD inline void elem_terms(const DevLayout &L, const Segment &sg, int64_t e, 
     case RF_SPIN_D200_4: {  // spin.cc:396-469
         const int g = sg.dim;
         const int64_t G = 2 * (int64_t)g;
-        const int R = (int)(e / G), C = (int)(e % G);
+        int R, C;
+        split_index(e, G, R, C);
         const int a = R >= g, bb = C >= g;
         const int IJ = R - a * g, KL = C - bb * g;
         const int pij = L.bab[L.bab_off[h] + IJ], pkl = L.bab[L.bab_off[h] + KL];
@@ -294,7 +319,8 @@ V2G_HD inline void elem_terms(const DevLayout &L, const Segment &sg, int64_t e, 
     }
     case RF_Q2AB: {  // q2.cc:813-849
         const int g = sg.dim;
-        const int IJ = (int)(e / g), KL = (int)(e % g);
+        int IJ, KL;
+        split_index(e, g, IJ, KL);
         const int pij = L.bab[L.bab_off[h] + IJ], pkl = L.bab[L.bab_off[h] + KL];
         const int i = P0(pij), j = P1(pij), k = P0(pkl), l = P1(pkl);
         f(L.xoff[XB_D2AB][h] + e, 1.0);
@@ -311,7 +337,8 @@ V2G_HD inline void elem_terms(const DevLayout &L, const Segment &sg, int64_t e, 
     }
     case RF_Q2AA: {  // q2.cc:851-948
         const int ga = sg.dim;
-        const int IJ = (int)(e / ga), KL = (int)(e % ga);
+        int IJ, KL;
+        split_index(e, ga, IJ, KL);
         const int pij = L.baa[L.baa_off[h] + IJ], pkl = L.baa[L.baa_off[h] + KL];
         const int i = P0(pij), j = P1(pij), k = P0(pkl), l = P1(pkl);
         const int wd = s ? XB_D1B : XB_D1A;
@@ -337,7 +364,8 @@ V2G_HD inline void elem_terms(const DevLayout &L, const Segment &sg, int64_t e, 
     }
     case RF_G2AB: {  // g2.cc:1067-1127
         const int g = sg.dim;
-        const int IJ = (int)(e / g), KL = (int)(e % g);
+        int IJ, KL;
+        split_index(e, g, IJ, KL);
         const int pij = L.bab[L.bab_off[h] + IJ], pkl = L.bab[L.bab_off[h] + KL];
         const int i = P0(pij), j = P1(pij), k = P0(pkl), l = P1(pkl);
         f(L.xoff[s ? XB_G2BA : XB_G2AB][h] + e, -1.0);
@@ -353,7 +381,8 @@ V2G_HD inline void elem_terms(const DevLayout &L, const Segment &sg, int64_t e, 
     case RF_G2AA: {  // g2.cc:1128-1257
         const int g = sg.dim;
         const int64_t G = 2 * (int64_t)g;
-        const int R = (int)(e / G), C = (int)(e % G);
+        int R, C;
+        split_index(e, G, R, C);
         const int ra = R >= g, cb = C >= g;
         const int IJ = R - ra * g, KL = C - cb * g;
         const int pij = L.bab[L.bab_off[h] + IJ], pkl = L.bab[L.bab_off[h] + KL];
@@ -511,7 +540,8 @@ V2G_HD inline double atu_d2ab(const DevLayout &L, const Segment &sg, int64_t e,
     const int amo = L.amo, h = sg.h, g = sg.dim;
     const int *gab = L.gab, *gaa = L.gaa, *sym = L.sym;
     auto GAB = [&](int p, int q) -> int64_t { return gab[p * amo + q]; };
-    const int IJ = (int)(e / g), KL = (int)(e % g);
+    int IJ, KL;
+        split_index(e, g, IJ, KL);
     const int pij = L.bab[L.bab_off[h] + IJ], pkl = L.bab[L.bab_off[h] + KL];
     const int p = P0(pij), q = P1(pij), r = P0(pkl), s = P1(pkl);
     double v = 0.0;
@@ -589,7 +619,8 @@ V2G_HD inline double atu_d2aa(const DevLayout &L, const Segment &sg, int64_t e,
 {
     const int amo = L.amo, h = sg.h, ga = sg.dim, sp = sg.sub;
     const int *gab = L.gab, *sym = L.sym;
-    const int IJ = (int)(e / ga), KL = (int)(e % ga);
+    int IJ, KL;
+        split_index(e, ga, IJ, KL);
     const int pij = L.baa[L.baa_off[h] + IJ], pkl = L.baa[L.baa_off[h] + KL];
     const int p1 = P0(pij), p2 = P1(pij), r1 = P0(pkl), r2 = P1(pkl);
     double v = 0.0;
@@ -674,7 +705,8 @@ V2G_HD inline double atu_self(const DevLayout &L, const Segment &sg, int64_t e,
     double v = sg.coef * y[sg.aux + e];
     if (sg.sub == 1) {  // G2ba of irrep 0 also appears in the maximal-spin rows
         const int g = sg.dim;
-        const int IJ = (int)(e / g), KL = (int)(e % g);
+        int IJ, KL;
+        split_index(e, g, IJ, KL);
         const int pij = L.bab[L.bab_off[0] + IJ], pkl = L.bab[L.bab_off[0] + KL];
         if (P0(pkl) == P1(pkl)) v += y[L.rowoff[RF_MAXSPIN][0][0] + IJ];
         if (P0(pij) == P1(pij)) v += y[L.rowoff[RF_MAXSPIN][1][0] + KL];

@@ -80,7 +80,8 @@ class V2RDMSolver:
                             CG_CONVERGENCE=1e-9, MAXITER=10000, CG_MAXITER=10000,
                             MU_UPDATE_FREQUENCY=500, PRECONDITIONER="NONE", FLEXIBLE_CG=False,
                             MLP_TRAIN_ITERS=30, MLP_TRAIN_BATCH=16384, MLP_TRAIN_STEPS=10,
-                            MLP_TRAIN_LR=2e-3, EIG_WARM_SWEEPS=0)
+                            MLP_TRAIN_LR=2e-3, EIG_WARM_SWEEPS=1, EIG_INNER_SWEEPS=1,
+                            CG_EXTRAPOLATE=True)
         if options:
             self.options.update({k.upper(): v for k, v in options.items()})
         self.x = self.z = self.y = self.c = self.b = None
@@ -265,7 +266,8 @@ class V2RDMSolver:
                         int(o["MU_UPDATE_FREQUENCY"]), pc, int(bool(o["FLEXIBLE_CG"])),
                         int(bool(verbose)), float(max_seconds), int(o["MLP_TRAIN_ITERS"]),
                         int(o["MLP_TRAIN_BATCH"]), int(o["MLP_TRAIN_STEPS"]),
-                        float(o["MLP_TRAIN_LR"]), int(o["EIG_WARM_SWEEPS"]))
+                        float(o["MLP_TRAIN_LR"]), int(o["EIG_WARM_SWEEPS"]),
+                        int(o["EIG_INNER_SWEEPS"]), int(bool(o["CG_EXTRAPOLATE"])))
         st = _lib.Stats()
         self._sync()
         check(self.lib.v2g_bpsdp(self.h, ctypes.byref(p), _ptr(self.c), _ptr(self.b),
