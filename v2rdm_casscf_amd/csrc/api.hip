@@ -465,9 +465,9 @@ void v2g_solver::update_xz(double mu, const double *y, const double *c, double *
             std::vector<v2g::GemmDesc> g1, g2;
             for (const auto &b : large) {
                 g1.push_back({Mw.p + b.woff, Vw.p + b.woff, Tw.p + b.woff, b.N, b.N, b.N, b.N, b.N,
-                              b.N, 0, 0});
+                              b.N, 0, 0, 0});
                 g2.push_back({Vw.p + b.woff, Tw.p + b.woff, Aw.p + b.woff, b.N, b.N, b.N, b.N, b.N,
-                              b.N, 1, 0});
+                              b.N, 1, 0, 1});
             }
             d_gemm_warm1.upload(g1);
             d_gemm_warm2.upload(g2);
@@ -494,7 +494,8 @@ void v2g_solver::update_xz(double mu, const double *y, const double *c, double *
     std::vector<v2g::GemmDesc> g3;
     int maxn = 0;
     for (const auto &b : large) {
-        g3.push_back({Tw.p + b.woff, Tw.p + b.woff, x + b.xoff, b.n, b.n, b.N, b.N, b.N, b.n, 0, 1});
+        g3.push_back({Tw.p + b.woff, Tw.p + b.woff, x + b.xoff, b.n, b.n, b.N, b.N, b.N, b.n, 0, 1,
+                      1});
         maxn = std::max(maxn, b.n);
     }
     d_gemm_rec.upload(g3);

@@ -117,27 +117,53 @@ __device__ int lds_jacobi(double *S, double *Q, double tol, int maxsweeps, int *
 //    touches LDS.  That leaves one 64 x 65 FP64 array of LDS per workgroup.
 // In step r of the circle method, lane l is paired with 63 <-> r and
 // l <-> (2r - l) mod 63 otherwise; kk(r, l) is the pair's slot.
+// x mod 63 for 0 <= x < 189 without a division
+__device__ __forceinline__ int mod63(int x)
+{
+    x = x >= 2 * (kP - 1) ? x - 2 * (kP - 1) : x;
+    return x >= kP - 1 ? x - (kP - 1) : x;
+}
+
 __device__ __forceinline__ int circle_partner(int r, int l)
 {
     if (l == kP - 1) return r;
     if (l == r) return kP - 1;
-    return (2 * r - l + 2 * (kP - 1)) % (kP - 1);
+    return mod63(2 * r - l + (kP - 1));
 }
 
 __device__ __forceinline__ int circle_slot(int r, int l)
 {
     if (l == kP - 1 || l == r) return 0;
-    const int d = (l - r + (kP - 1)) % (kP - 1);
-    return d <= kP / 2 - 1 ? d : (r - l + (kP - 1)) % (kP - 1);
+    const int d = mod63(l - r + (kP - 1));
+    return d <= kP / 2 - 1 ? d : mod63(r - l + (kP - 1));
+}
+
+// the (smaller, larger) index pair of slot k in step r, as circle_pair(kP, r, k)
+__device__ __forceinline__ void slot_pair(int r, int k, int &p, int &q)
+{
+    int a, b;
+    if (k == 0) { a = kP - 1; b = r; }
+    else { a = mod63(r + k); b = mod63(r - k + (kP - 1)); }
+    p = a < b ? a : b;
+    q = a < b ? b : a;
 }
 
 constexpr int kUT = (kP / 2) * (kP / 2 + 1) / 2;   // 528 pair blocks with P <= R
+constexpr int kUTper = (kUT + kT - 1) / kT;         // blocks per thread
 
 __device__ int pair_jacobi(double *S, double (&q)[kP / 4], double tol, int maxsweeps,
                            const unsigned short *ut, int *sh_cnt, double *sh_c, double *sh_s,
                            int *sh_pa, int *sh_pb)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    // this thread's upper-triangle pair blocks (fixed for the whole solve)
+    int myP[kUTper], myR[kUTper];
+#pragma unroll
+    for (int u = 0; u < kUTper; u++) {
+        const int w = threadIdx.x + u * kT;
+        myP[u] = w < kUT ? (ut[w] & 0xff) : -1;
+        myR[u] = w < kUT ? (ut[w] >> 8) : -1;
+    }
     int total = 0;
     for (int sweep = 0; sweep < maxsweeps; sweep++) {
         if (threadIdx.x == 0) *sh_cnt = 0;
@@ -172,22 +198,22 @@ __device__ int pair_jacobi(double *S, double (&q)[kP / 4], double tol, int maxsw
             // to kUTper blocks; all their LDS reads are issued before any
             // write (the blocks are disjoint), so their latencies overlap.
             {
-                constexpr int kUTper = (kUT + kT - 1) / kT;
                 int addr[kUTper][4];
                 double bv[kUTper][4], cf[kUTper][4];
                 bool act[kUTper];
 #pragma unroll
                 for (int u = 0; u < kUTper; u++) {
-                    const int w = threadIdx.x + u * kT;
                     act[u] = false;
-                    if (w < kUT) {
-                        const int P = ut[w] & 0xff, R = ut[w] >> 8;
+                    if (myP[u] >= 0) {
+                        const int P = myP[u], R = myR[u];
                         cf[u][0] = sh_c[P];
                         cf[u][1] = sh_s[P];
                         cf[u][2] = sh_c[R];
                         cf[u][3] = sh_s[R];
                         act[u] = !(cf[u][1] == 0.0 && cf[u][3] == 0.0);
-                        const int p0 = sh_pa[P], p1 = sh_pb[P], r0 = sh_pa[R], r1 = sh_pb[R];
+                        int p0, p1, r0, r1;
+                        slot_pair(r, P, p0, p1);
+                        slot_pair(r, R, r0, r1);
                         auto ad = [](int i, int j) { return i < j ? i * kLd + j : j * kLd + i; };
                         addr[u][0] = ad(p0, r0);
                         addr[u][1] = ad(p0, r1);
@@ -401,27 +427,29 @@ k_pair_eig(const EigBlock *__restrict__ blocks, int round, const double *__restr
     }
 }
 
-// 64x64x64 product on FP64 MFMA: out = op(X) Y with X, Y in LDS (pitch kLd);
-// transX: X is read transposed.  Wave w owns the 32x32 output sub-tile
-// (rows 32 (w/2), cols 32 (w%2)); v_mfma_f64_16x16x4f64 operand layout:
-// A[i][k] from lane i + 16k, B[k][j] from lane j + 16k, D[i][j] in lane
-// j + 16 (i%4), element i/4 (one FP64 row per accumulator register).
-__device__ __forceinline__ void mfma_tile(const double *X, const double *Y, bool transX,
-                                          v4d (&acc)[2][2])
+// 64x64x64 product on FP64 MFMA: out = op(X) Y with row pitches ldx, ldy
+// (an operand may sit in LDS or in global memory); transX: X is read
+// transposed.  Wave w owns the 32x32 output sub-tile (rows 32 (w/2), cols
+// 32 (w%2)); v_mfma_f64_16x16x4f64 operand layout: A[i][k] from lane i + 16k,
+// B[k][j] from lane j + 16k, D[i][j] in lane j + 16 (i%4), element i/4 (one
+// FP64 row per accumulator register).
+__device__ __forceinline__ void mfma_tile(const double *X, int ldx, const double *Y, int ldy,
+                                          bool transX, v4d (&acc)[2][2])
 {
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
     const int li = lane & 15, lk = lane >> 4;
     for (int a = 0; a < 2; a++)
         for (int b = 0; b < 2; b++) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
     for (int k0 = 0; k0 < kP; k0 += 4) {
         const int k = k0 + lk;
         double av[2], bv[2];
         for (int a = 0; a < 2; a++) {
             const int row = wr + a * 16 + li;
-            av[a] = transX ? X[k * kLd + row] : X[row * kLd + k];
+            av[a] = transX ? X[k * ldx + row] : X[row * ldx + k];
         }
-        for (int b = 0; b < 2; b++) bv[b] = Y[k * kLd + wc + b * 16 + li];
+        for (int b = 0; b < 2; b++) bv[b] = Y[k * ldy + wc + b * 16 + li];
         for (int a = 0; a < 2; a++)
             for (int b = 0; b < 2; b++)
                 acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
@@ -444,8 +472,10 @@ __global__ void __launch_bounds__(kT)
 k_apply(const EigBlock *__restrict__ blocks, int round, double *__restrict__ Aw,
         double *__restrict__ Vw, const double *__restrict__ Qbuf, const int *__restrict__ active)
 {
+    // only the tile lives in LDS; the pair rotations Q_P, Q_R are read from
+    // global memory (L2-resident: each one serves a row of tiles), which
+    // halves the LDS footprint and doubles the workgroups per CU
     __shared__ double T[kP * kLd];
-    __shared__ double Qs[kP * kLd];
     const EigBlock B = blocks[blockIdx.y];
     const int nbk = B.N / 32, np = nbk / 2;
     if (round >= nbk - 1) return;
@@ -476,21 +506,18 @@ k_apply(const EigBlock *__restrict__ blocks, int round, double *__restrict__ Aw,
         const int i = w / kP, j = w % kP;
         T[i * kLd + j] = M[row_of(i) * B.N + pair_index(ra, rb, j)];
     }
+    __syncthreads();
     v4d acc[2][2];
     if (actP) {
         const double *QP = Qbuf + B.qoff + (int64_t)P * kP * kP;
-        for (int w = threadIdx.x; w < kP * kP; w += kT) Qs[(w / kP) * kLd + (w % kP)] = QP[w];
-        __syncthreads();
-        mfma_tile(Qs, T, true, acc);      // Q_P^T T
+        mfma_tile(QP, kP, T, kLd, true, acc);      // Q_P^T T
         __syncthreads();
         store_tile(T, acc);
+        __syncthreads();
     }
     if (actR) {
         const double *QR = Qbuf + B.qoff + (int64_t)R * kP * kP;
-        __syncthreads();
-        for (int w = threadIdx.x; w < kP * kP; w += kT) Qs[(w / kP) * kLd + (w % kP)] = QR[w];
-        __syncthreads();
-        mfma_tile(T, Qs, false, acc);     // T Q_R
+        mfma_tile(T, kLd, QR, kP, false, acc);     // T Q_R
         __syncthreads();
         store_tile(T, acc);
     }

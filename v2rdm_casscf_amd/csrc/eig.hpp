@@ -23,6 +23,7 @@ struct GemmDesc {
     int M, N, K;
     int lda, ldb, ldc;
     int ta, tb;     // 1: use the transpose of A / B
+    int sym;        // 1: C is symmetric (M == N); only upper tiles are computed
 };
 
 void eig_small(const EigBlock *blocks, int nblocks, const double *aty, const double *c, double *x,
