@@ -20,8 +20,10 @@ Also reported on the same JSON line:
   roofline      the A^T.y + A.x map kernels (the CG operator), timed with HIP
                 events on the engine's stream: algorithmic bytes per launch
                 8 (dimx + ncon) / mean launch time, against 8 TB/s HBM;
-  eig           the Update_xz eigen split timed the same way, with its
-                algorithmic FP64 flops sum (16/3) N^3 against 78.6 TFLOP/s;
+  eig           the Update_xz eigen split as the loop runs it (wall time per
+                outer iteration on the engine stream), against the algorithmic FP64
+                flops of a full eigendecomposition, sum (16/3) N^3, and 78.6 TFLOP/s;
+                one converged warm call, timed with HIP events, is reported beside it;
   cpu_baseline  the CPU oracle (OpenMP maps + MKL dsyev) on the host, on a
                 bounded sample (one CG operator application and one Update_xz
                 timed; an outer iteration projected with the GPU run's mean CG
@@ -55,22 +57,37 @@ def _workload(name):
         return dict(name=f"H{n} STO-3G r=1.8 bohr D2h ({n}e,{n}o) DQG+spin", amopi=d["amopi"],
                     na=n // 2, nb=n // 2, h1=d["h1"], eri=d["eri"], enuc=d["enuc"])
     if name.startswith("syn"):
-        n = int(name[3:])
-        h1, eri = synthetic_cholesky(n, 0)
-        return dict(name=f"synthetic Cholesky n={n} C1 ({n}e,{n}o) DQG+spin", amopi=[n],
+        # synN or synNsK: n = N, seed K (default 0), L = 3n Cholesky vectors
+        body = name[3:]
+        n, seed = (int(body.split("s")[0]), int(body.split("s")[1])) if "s" in body else (int(body), 0)
+        h1, eri = synthetic_cholesky(n, seed)
+        return dict(name=f"synthetic Cholesky n={n} seed {seed} C1 ({n}e,{n}o)", amopi=[n],
                     na=n // 2, nb=n // 2, h1=h1, eri=eri, enuc=0.0)
     raise ValueError(name)
+
+
+def _map_traffic(workload):
+    """HBM bytes per A^T.y / A.x launch from the committed counter summary
+    (rocprofv3 FETCH_SIZE / WRITE_SIZE runs, scaled by a plain copy kernel)."""
+    path = os.path.join(ROOT, "profiles", f"maps_{workload}_counters.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch"):
+            kv = dict(tok.split("=") for tok in line.split()[1:])
+            return {k: int(v) for k, v in kv.items()}
+    return None
 
 
 def _eig_flops(dims):
     return sum((16.0 / 3.0) * float(n) ** 3 for n in dims)
 
 
-def cpu_baseline(w, gpu_cg_per_outer, seconds_budget=25.0):
+def cpu_baseline(w, gpu_cg_per_outer, positivity="DQG"):
     """Time the oracle's CG operator and Update_xz on a bounded sample."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py as orc
-    L = orc.Layout(w["amopi"], w["na"], w["nb"], None, "DQG")
+    L = orc.Layout(w["amopi"], w["na"], w["nb"], None, positivity)
     c = L.build_c(w["h1"], w["eri"])
     x, z, y = L.guess_random()
     t0 = time.time()
@@ -118,6 +135,32 @@ def _maps_only(s, stream, reps, w):
     print(json.dumps(out))
 
 
+def _scan(args):
+    """Many independent H-chains at once (one handle and stream per chain)."""
+    from v2rdm_casscf_amd.bondscan import make_hchain_scan, solve_scan
+    pts = make_hchain_scan(args.scan, seed=0)
+    opts = dict(PRECONDITIONER=args.precond.upper(), EIG_WARM_SWEEPS=args.eig_sweeps,
+                EIG_INNER_SWEEPS=args.inner_sweeps, MAXITER=10 ** 6)
+
+    def progress(i, r):
+        if i < 0:
+            print(f"# {r['stage']}", file=sys.stderr, flush=True)
+            return
+        print(f"# point {i}: H{r['n']} r={r['spacing_angstrom']:.3f} E={r['energy']:.8f} "
+              f"oiter={r['oiter']} converged={r['converged']}", file=sys.stderr, flush=True)
+
+    res, wall = solve_scan(pts, positivity=args.positivity, workers=len(pts), options=opts,
+                           max_seconds=args.converge, progress=progress)
+    conv = sum(int(r["converged"]) for r in res)
+    tot = sum(r["oiter"] for r in res)
+    print(json.dumps(dict(mode="bond-scan", points=len(pts), converged=conv, seconds=round(wall, 2),
+                          outer_iterations=tot, outer_iterations_per_s=round(tot / wall, 1),
+                          max_outer=max(r["oiter"] for r in res),
+                          per_point=[dict(n=r["n"], r=round(r["spacing_angstrom"], 4),
+                                          energy=round(r["energy"], 8), oiter=r["oiter"],
+                                          converged=r["converged"]) for r in res])))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,6 +183,11 @@ def main():
     ap.add_argument("--converge", type=float, default=0.0,
                     help="instead of timing K steps: run from the RANDOM guess to R/E_CONVERGENCE "
                          "1e-6 (stopping after this many seconds) and report time-to-converge")
+    ap.add_argument("--scan", type=int, default=0,
+                    help="batched bond scan: this many H-chains (n in 8..16, spacing 0.7-3.0 A, "
+                         "seed 0) solved concurrently to R/E_CONVERGENCE 1e-6; --converge caps it")
+    ap.add_argument("--cpu-iters", type=int, default=0,
+                    help="with --converge: time this many CPU-oracle outer iterations too")
     ap.add_argument("--maps-only", action="store_true",
                     help="only the A^T.y / A.x kernels (map-reps calls each) and a device copy "
                          "of 8 (dimx + ncon) bytes as the traffic reference, for counter runs")
@@ -151,6 +199,9 @@ def main():
 
     import torch
     from v2rdm_casscf_amd import V2RDMSolver
+    if args.scan > 0:
+        _scan(args)
+        return
 
     dev = torch.device("cuda:0")
     w = _workload(args.workload)
@@ -186,12 +237,26 @@ def main():
         secs = time.perf_counter() - t0
         keep = ("oiter", "cg_iters", "converged", "ep", "ed", "mu", "t_cg", "t_xz",
                 "precond_active", "fallback", "cg_iters_train", "train_outer", "mlp_loss")
-        print(json.dumps(dict(mode="time-to-converge", workload=w["name"],
-                              positivity=args.positivity, precond=args.precond,
-                              dimx=s.dimx, ncon=s.nconstraints, seconds=round(secs, 3),
-                              energy=st["energy_primal"] + w["enuc"],
-                              energy_dual=st["energy_dual"] + w["enuc"],
-                              **{k: st[k] for k in keep if k in st})))
+        rec = dict(mode="time-to-converge", workload=w["name"], positivity=args.positivity,
+                   precond=args.precond, dimx=s.dimx, ncon=s.nconstraints, seconds=round(secs, 3),
+                   energy=st["energy_primal"] + w["enuc"], energy_dual=st["energy_dual"] + w["enuc"],
+                   **{k: st[k] for k in keep if k in st})
+        if args.cpu_iters > 0:
+            # the CPU oracle (the reference's plain CG, LAPACK eigensolver) on the
+            # same guess for a few outer iterations: seconds per outer iteration,
+            # and the time-to-converge it implies at the GPU run's iteration count
+            sys.path.insert(0, os.path.join(ROOT, "oracle"))
+            import oracle_py as orc
+            Lc = orc.Layout(w["amopi"], w["na"], w["nb"], None, args.positivity)
+            cc = Lc.build_c(w["h1"], w["eri"])
+            xc, zc, yc = Lc.guess_random()
+            t0 = time.perf_counter()
+            rc = Lc.bpsdp(cc, Lc.build_b(), xc, zc, yc, maxiter=args.cpu_iters)
+            cpu_it = (time.perf_counter() - t0) / max(1, rc["oiter"])
+            rec["cpu_baseline"] = dict(seconds_per_outer=round(cpu_it, 4), cores=orc.num_threads(),
+                                       kind="port", sample=f"{rc['oiter']} oracle outer iterations",
+                                       projected_seconds=round(cpu_it * st["oiter"], 1))
+        print(json.dumps(rec))
         return
 
     def timed_leg(solver):
@@ -273,14 +338,23 @@ def main():
                     unit="GB/s", traffic=None, t_atu_ms=round(1e3 * t_atu, 4),
                     t_au_ms=round(1e3 * t_au, 4), bytes_per_launch=bytes_map)
     roofline["frac"] = round(roofline["achieved"] / HBM_PEAK_GBS, 4)
-    eig = dict(bound="mfma", achieved=round(eflops / t_xz / 1e12, 3), peak=FP64_PEAK_TFLOPS,
-               unit="TFLOP/s", t_update_xz_ms=round(1e3 * t_xz, 3), flops=eflops,
-               sweeps=st["eig_sweeps"])
+    tr = _map_traffic(args.workload)
+    if tr and "k_atu" in tr and "k_au" in tr:
+        roofline["traffic"] = int((tr["k_atu"] + tr["k_au"]) / 2)
+        roofline["traffic_k_atu"], roofline["traffic_k_au"] = tr["k_atu"], tr["k_au"]
+    # the eigen split as the loop runs it (warm, tracked): its wall time per outer
+    # iteration on the engine stream, against the flops of a full eigendecomposition;
+    # the converged warm call measured with events above is reported beside it
+    t_xz_loop = st["t_xz"] / max(1, nsteps)
+    eig = dict(bound="mfma", achieved=round(eflops / t_xz_loop / 1e12, 3), peak=FP64_PEAK_TFLOPS,
+               unit="TFLOP/s", t_update_xz_loop_ms=round(1e3 * t_xz_loop, 3), flops=eflops,
+               warm_sweeps_per_call=args.eig_sweeps, inner_sweeps=args.inner_sweeps,
+               t_update_xz_converged_ms=round(1e3 * t_xz, 3))
     eig["frac"] = round(eig["achieved"] / FP64_PEAK_TFLOPS, 4)
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(w, cg_per_outer)
+            cpu = cpu_baseline(w, cg_per_outer, args.positivity)
         except Exception as e:  # reported, not fatal
             cpu = dict(error=str(e))
     value = nsteps / elapsed

@@ -26,8 +26,10 @@ def make_hchain_scan(count=64, seed=0, sizes=(8, 10, 12, 14, 16), lo=0.7, hi=3.0
 
 
 def solve_scan(points, positivity="DQG", workers=16, device="cuda:0", options=None,
-               max_seconds=0.0):
-    """Solve every (n, spacing_angstrom) chain; returns (results, wall seconds)."""
+               max_seconds=0.0, progress=None):
+    """Solve every (n, spacing_angstrom) chain; returns (results, wall seconds).
+    progress(i, result) is called as each point finishes (and, with running=True
+    in the result, at most every 20 s while points are iterating)."""
     hams = {}
     with cf.ThreadPoolExecutor(max_workers=8) as ex:
         futs = {ex.submit(hydrogen_chain, n, r * ANGSTROM_TO_BOHR): i
@@ -44,10 +46,27 @@ def solve_scan(points, positivity="DQG", workers=16, device="cuda:0", options=No
         solvers[i].BuildConstraints()
         solvers[i].Guess()
 
+    if progress:
+        progress(-1, dict(stage=f"{len(points)} problems set up"))
+    last = [time.perf_counter()]
+
     def run(i):
         d = hams[i]
+        if progress:
+            def report(oiter, st):
+                # at most one line every 20 s across all points
+                now = time.perf_counter()
+                if now - last[0] > 20.0:
+                    last[0] = now
+                    progress(i, dict(st, n=points[i][0], spacing_angstrom=points[i][1],
+                                     energy=st["energy_primal"] + d["enuc"], converged=0,
+                                     running=True))
+                return False
+            solvers[i].set_iter_callback(report)
         out = solvers[i].compute_energy(enuc=d["enuc"], guess=False, max_seconds=max_seconds)
         out.update(n=points[i][0], spacing_angstrom=points[i][1], e_rhf=d["e_rhf"])
+        if progress:
+            progress(i, out)
         return i, out
 
     t0 = time.perf_counter()

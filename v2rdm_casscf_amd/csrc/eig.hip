@@ -245,11 +245,14 @@ __device__ int pair_jacobi(double *S, double (&q)[kP / 4], double tol, int maxsw
                 const int k = circle_slot(r, lane);
                 const double c = sh_c[k], sn = sh_s[k];
                 const bool first = lane < pl;
+                // all shuffles first, then the arithmetic: one wait for the
+                // whole batch instead of one per row
+                double o[kP / 4];
 #pragma unroll
-                for (int i = 0; i < kP / 4; i++) {
-                    const double o = __shfl(q[i], pl, 64);
-                    q[i] = first ? c * q[i] - sn * o : sn * o + c * q[i];
-                }
+                for (int i = 0; i < kP / 4; i++) o[i] = __shfl(q[i], pl, 64);
+#pragma unroll
+                for (int i = 0; i < kP / 4; i++)
+                    q[i] = first ? c * q[i] - sn * o[i] : sn * o[i] + c * q[i];
             }
             __syncthreads();
         }
