@@ -128,9 +128,12 @@ void emu_Au(void *h, const double *x, double *Ax)
     const int64_t ncon = E->L.ncon;
     std::vector<char> seen(ncon, 0);
     for (const Segment &sg : E->L.rowsegs) {
-        for (int64_t e = 0; e < sg.count; e++) {
+        for (int64_t pos = 0; pos < sg.count; pos++) {
+            // the row the kernels compute at this position (a permutation of
+            // the segment; duplicates or gaps count as violations)
+            const int64_t e = sg.wave ? pos : rows::row_of(E->D, sg, pos);
             const int64_t r = sg.start + e;
-            if (r < 0 || r >= ncon || seen[r]) { E->violations++; continue; }
+            if (e < 0 || e >= sg.count || r < 0 || r >= ncon || seen[r]) { E->violations++; continue; }
             seen[r] = 1;
             double v = 0.0;
             if (sg.wave) {
@@ -159,9 +162,10 @@ void emu_ATu(void *h, const double *y, double *ATy)
     const int64_t dimx = E->L.dimx;
     std::vector<char> seen(dimx, 0);
     for (const Segment &sg : E->L.xsegs) {
-        for (int64_t e = 0; e < sg.count; e++) {
+        for (int64_t pos = 0; pos < sg.count; pos++) {
+            const int64_t e = sg.wave ? pos : rows::entry_of(E->D, sg, pos);
             const int64_t i = sg.start + e;
-            if (i < 0 || i >= dimx || seen[i]) { E->violations++; continue; }
+            if (e < 0 || e >= sg.count || i < 0 || i >= dimx || seen[i]) { E->violations++; continue; }
             seen[i] = 1;
             double v = 0.0;
             if (sg.wave) {

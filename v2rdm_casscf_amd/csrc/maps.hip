@@ -110,19 +110,20 @@ k_au(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
     } else {
         // kPerThread independent rows per thread: their gathers overlap in flight
         double v[kPerThread];
+        int64_t row[kPerThread];
 #pragma unroll
         for (int u = 0; u < kPerThread; u++) {
-            const int64_t e = t * kElemTile + u * kBlock + threadIdx.x;
+            const int64_t pos = t * kElemTile + u * kBlock + threadIdx.x;
             Gather f{x, 0.0};
-            if (e < sg.count) elem_terms(L, sg, e, f);
+            row[u] = pos < sg.count ? row_of(L, sg, pos) : -1;
+            if (row[u] >= 0) elem_terms(L, sg, row[u], f);
             v[u] = f.acc;
         }
 #pragma unroll
         for (int u = 0; u < kPerThread; u++) {
-            const int64_t e = t * kElemTile + u * kBlock + threadIdx.x;
-            if (e < sg.count) {
-                Ax[sg.start + e] = v[u];
-                if (dotv) contrib += v[u] * dotv[sg.start + e];
+            if (row[u] >= 0) {
+                Ax[sg.start + row[u]] = v[u];
+                if (dotv) contrib += v[u] * dotv[sg.start + row[u]];
             }
         }
     }
@@ -171,8 +172,9 @@ k_row_norm2(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
         if (lane == 0) d[sg.start + item] = v;
     } else {
         for (int u = 0; u < kPerThread; u++) {
-            const int64_t e = t * kElemTile + u * kBlock + threadIdx.x;
-            if (e >= sg.count) return;
+            const int64_t pos = t * kElemTile + u * kBlock + threadIdx.x;
+            if (pos >= sg.count) return;
+            const int64_t e = row_of(L, sg, pos);
             Collect f;
             elem_terms(L, sg, e, f);
             d[sg.start + e] = f.norm2();
@@ -203,23 +205,23 @@ k_atu(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
         return;
     }
     double v[kPerThread];
+    int64_t ent[kPerThread];
 #pragma unroll
     for (int u = 0; u < kPerThread; u++) {
-        const int64_t e = t * kElemTile + u * kBlock + threadIdx.x;
+        const int64_t pos = t * kElemTile + u * kBlock + threadIdx.x;
         v[u] = 0.0;
-        if (e < sg.count) {
+        ent[u] = pos < sg.count ? entry_of(L, sg, pos) : -1;
+        if (ent[u] >= 0) {
             switch (sg.kind) {
-            case XS_D2AB: v[u] = atu_d2ab(L, sg, e, y); break;
-            case XS_D2AA: v[u] = atu_d2aa(L, sg, e, y); break;
-            default: v[u] = atu_self(L, sg, e, y); break;
+            case XS_D2AB: v[u] = atu_d2ab(L, sg, ent[u], y); break;
+            case XS_D2AA: v[u] = atu_d2aa(L, sg, ent[u], y); break;
+            default: v[u] = atu_self(L, sg, ent[u], y); break;
             }
         }
     }
 #pragma unroll
-    for (int u = 0; u < kPerThread; u++) {
-        const int64_t e = t * kElemTile + u * kBlock + threadIdx.x;
-        if (e < sg.count) ATy[sg.start + e] = v[u];
-    }
+    for (int u = 0; u < kPerThread; u++)
+        if (ent[u] >= 0) ATy[sg.start + ent[u]] = v[u];
 }
 
 // T2 rows: self entry written, delta-selected D2 / D1 entries accumulated

@@ -531,6 +531,53 @@ struct SumSq {
 };
 
 // ----------------------------------------------------------------------------
+// processing order.  Rows (or primal entries) of a block are written where
+// they belong, but the kernels may visit them in another order.  In the G2ba
+// and G2 super-block rows (and in the D2ab entries of A^T), the D2ab / G2ba
+// operand's column index runs over the second orbital of a pair, so one row
+// pair touches single doubles spread over many cache lines; the row pair with
+// the transposed orbitals touches the neighbouring doubles of the same lines.
+// Visiting the row pairs in transposed order puts those neighbours in
+// consecutive workgroups, so the lines are reused from L2.
+// ----------------------------------------------------------------------------
+
+// ab pair index of the transposed pair of the pair at position pos of irrep h
+V2G_HD inline int ab_transposed(const DevLayout &L, int h, int pos)
+{
+    const int pr = L.bab[L.bab_off[h] + pos];
+    return L.gab[P1(pr) * L.amo + P0(pr)];
+}
+
+// row offset (within its segment) visited at position pos of an A.x segment
+V2G_HD inline int64_t row_of(const DevLayout &L, const Segment &sg, int64_t pos)
+{
+    if (sg.kind == RF_G2AB && sg.sub == 1) {
+        int IJ, KL;
+        split_index(pos, sg.dim, IJ, KL);
+        return (int64_t)ab_transposed(L, sg.h, IJ) * sg.dim + KL;
+    }
+    if (sg.kind == RF_G2AA) {
+        const int g = sg.dim;
+        int R, C;
+        split_index(pos, 2 * (int64_t)g, R, C);
+        const int half = R >= g;
+        return ((int64_t)half * g + ab_transposed(L, sg.h, R - half * g)) * (2 * (int64_t)g) + C;
+    }
+    return pos;
+}
+
+// primal entry (within its segment) visited at position pos of an A^T.y segment
+V2G_HD inline int64_t entry_of(const DevLayout &L, const Segment &sg, int64_t pos)
+{
+    if (sg.kind == XS_D2AB) {
+        int PQ, RS;
+        split_index(pos, sg.dim, PQ, RS);
+        return (int64_t)ab_transposed(L, sg.h, PQ) * sg.dim + RS;
+    }
+    return pos;
+}
+
+// ----------------------------------------------------------------------------
 // A^T.y gathers
 // ----------------------------------------------------------------------------
 
