@@ -69,6 +69,24 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t i, int64_t nb)
     return x * q + (x < rem ? x : rem) + i / kXcds;
 }
 
+// Chunked variant for launches whose segments differ in cost: XCD x runs the
+// chunks x, x + 8, x + 16, ... of kChunk consecutive tiles, which keeps
+// neighbouring tiles on one L2 and the XCDs' loads even.  The grid is padded
+// to a multiple of 8 kChunk; positions past the last tile do nothing.
+constexpr int kChunk = 16;
+
+__device__ __forceinline__ int64_t xcd_chunk_tile(int64_t i)
+{
+    const int64_t x = i % kXcds, l = i / kXcds;
+    return ((l / kChunk) * kXcds + x) * kChunk + (l % kChunk);
+}
+
+inline int64_t chunk_grid(int64_t ntiles)
+{
+    const int64_t c = (int64_t)kXcds * kChunk;
+    return (ntiles + c - 1) / c * c;
+}
+
 __device__ __forceinline__ int find_segment(const int64_t *__restrict__ tile_start, int nseg,
                                             int64_t b)
 {
@@ -185,14 +203,15 @@ k_row_norm2(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
 __global__ void __launch_bounds__(kBlock)
 k_atu(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
       const int64_t *__restrict__ tile_start, int nseg, const double *__restrict__ y,
-      double *__restrict__ ATy, const double *__restrict__ done)
+      double *__restrict__ ATy, const double *__restrict__ done, int64_t ntiles)
 {
     if (done && *done != 0.0) return;
     const DevLayout &L = *Lp;
-    // blockIdx order (not xcd_tile): the primal segments differ widely in cost
-    // (closed-form D2 gathers vs copies), and contiguous per-XCD ranges would
-    // leave some XCDs with all of the expensive tiles
-    const int64_t tile = blockIdx.x;
+    // chunked (not contiguous) XCD ranges: the primal segments differ widely in
+    // cost (closed-form D2 gathers vs copies), and contiguous per-XCD ranges
+    // would leave some XCDs with all of the expensive tiles
+    const int64_t tile = xcd_chunk_tile(blockIdx.x);
+    if (tile >= ntiles) return;
     const int si = find_segment(tile_start, nseg, tile);
     const Segment sg = segs[si];
     const int64_t t = tile - tile_start[si];
@@ -265,8 +284,8 @@ void launch_row_norm2(const MapPlan &p, double *d, hipStream_t st)
 void launch_atu(const MapPlan &p, const double *y, double *ATy, hipStream_t st, const double *done)
 {
     if (p.x_tiles > 0)
-        hipLaunchKernelGGL(k_atu, dim3((unsigned)p.x_tiles), dim3(kBlock), 0, st, p.dlay,
-                           p.xsegs, p.x_tile_start, p.nxsegs, y, ATy, done);
+        hipLaunchKernelGGL(k_atu, dim3((unsigned)chunk_grid(p.x_tiles)), dim3(kBlock), 0, st, p.dlay,
+                           p.xsegs, p.x_tile_start, p.nxsegs, y, ATy, done, p.x_tiles);
     if (p.t2_tiles > 0)
         hipLaunchKernelGGL(k_atu_t2, dim3((unsigned)p.t2_tiles), dim3(kBlock), 0, st, p.dlay,
                            p.t2segs, p.t2_tile_start, p.nt2segs, y, ATy, done);
