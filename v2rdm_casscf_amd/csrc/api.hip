@@ -63,6 +63,9 @@ double now()
     return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
 }
 
+constexpr int kCgChunk = 8;                 // CG iterations between host checks (even)
+constexpr int64_t kGraphMaxRows = 4 << 20;  // replay captured chunks below this many rows
+
 }  // namespace
 
 struct v2g_solver {
@@ -105,9 +108,19 @@ struct v2g_solver {
     int adam_t = 0, batch_cap = 0;
     bool mlp_ready = false;
 
+    // CG iterations replayed as a captured HIP graph of kCgChunk iterations
+    // (problems up to kGraphMaxRows rows, where launch latency dominates).
+    // Opt-in (V2G_CG_GRAPHS=1): with many handles driven from concurrent host
+    // threads, a capture on one stream has made other threads' copies fail.
+    bool use_graphs = false;
+    hipGraphExec_t cg_exec = nullptr;
+    const double *cg_exec_y = nullptr, *cg_exec_minv = nullptr;
+    int cg_exec_flex = -1;
+
     explicit v2g_solver(const v2g_layout_desc &d);
     ~v2g_solver()
     {
+        if (cg_exec) (void)hipGraphExecDestroy(cg_exec);
         if (own) (void)hipStreamDestroy(own);
     }
     void ensure_jacobi();
@@ -139,6 +152,7 @@ v2g_solver::v2g_solver(const v2g_layout_desc &d)
     o.spin = d.constrain_spin;
     o.sz = d.constrain_sz;
     L.reset(new v2g::Layout(o));
+    if (const char *e = std::getenv("V2G_CG_GRAPHS")) use_graphs = std::atoi(e) != 0;
     HIPCHK(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
     st = own;
 
@@ -422,23 +436,55 @@ int v2g_solver::cg(const double *rhs, double *y, double tolv, int maxiter, int p
     v2g::cg_init(n, rhs, Ap.p, r.p, p.p, minv, nullptr, partial.p, scal.p, tolv, maxiter, st);
     if (r0_out)
         HIPCHK(hipMemcpyAsync(r0_out, r.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-    const int chunk = 8;
-    double hs[v2g::CG_NSLOTS];
-    int it = 0;
-    for (;;) {
-        for (int c = 0; c < chunk; c++, it++) {
-            const int par = it & 1;
+    // one chunk of kCgChunk (even) iterations; every kernel argument is fixed,
+    // the iteration number and the done flags live in scal
+    auto enqueue_chunk = [&]() {
+        for (int c = 0; c < kCgChunk; c++) {
+            const int par = c & 1;
             const double *done = scal.p + v2g::CG_DONE0 + par;
             v2g::launch_atu(plan, p.p, t.p, st, done);
             v2g::launch_au(plan, t.p, Ap.p, st, done, p.p, rowpart.p);   // Ap and p.Ap
             v2g::cg_dot_from_rows(plan.row_tiles, rowpart.p, partial.p, scal.p, par, st);
             v2g::cg_step(n, y, r.p, p.p, Ap.p, minv, nullptr, nullptr, partial.p, scal.p, par, st);
-            v2g::cg_finish_iter(n, r.p, p.p, minv, partial.p, scal.p, par, flexible, it, st);
+            v2g::cg_finish_iter(n, r.p, p.p, minv, partial.p, scal.p, par, flexible, st);
         }
+    };
+    hipGraphExec_t exec = nullptr;
+    if (use_graphs && n <= kGraphMaxRows) {
+        if (!(cg_exec && cg_exec_y == y && cg_exec_minv == minv && cg_exec_flex == flexible)) {
+            if (cg_exec) { (void)hipGraphExecDestroy(cg_exec); cg_exec = nullptr; }
+            // relaxed mode: the other host threads of a bond scan keep allocating,
+            // copying and synchronizing their own streams while this one captures
+            hipGraph_t g = nullptr;
+            bool ok = hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed) == hipSuccess;
+            if (ok) {
+                enqueue_chunk();
+                ok = hipStreamEndCapture(st, &g) == hipSuccess && g;
+            }
+            if (ok) ok = hipGraphInstantiate(&cg_exec, g, nullptr, nullptr, 0) == hipSuccess;
+            if (g) (void)hipGraphDestroy(g);
+            if (!ok) {            // capture unsupported here: stay with plain launches
+                cg_exec = nullptr;
+                use_graphs = false;
+                (void)hipGetLastError();
+            } else {
+                cg_exec_y = y;
+                cg_exec_minv = minv;
+                cg_exec_flex = flexible;
+            }
+        }
+        exec = cg_exec;
+    }
+    double hs[v2g::CG_NSLOTS];
+    int it = 0;
+    for (;;) {
+        if (exec) HIPCHK(hipGraphLaunch(exec, st));
+        else enqueue_chunk();
+        it += kCgChunk;
         HIPCHK(hipMemcpyAsync(hs, scal.p, sizeof(hs), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         if (hs[v2g::CG_DONE0 + (it & 1)] != 0.0) break;
-        if (it > maxiter + 2 * chunk) break;  // defensive: cannot happen
+        if (it > maxiter + 2 * kCgChunk) break;  // defensive: cannot happen
     }
     if (res) *res = hs[v2g::CG_NRM];
     return (int)hs[v2g::CG_ITER];

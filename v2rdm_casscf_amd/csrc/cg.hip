@@ -26,7 +26,7 @@ constexpr int kThreads = 256;
 
 enum : int { S_RZ0 = CG_RZ0, S_DONE0 = CG_DONE0, S_DONE1 = CG_DONE1, S_NRM = CG_NRM,
              S_ITER = CG_ITER, S_ALPHA = CG_ALPHA, S_BETA = CG_BETA, S_PAP = CG_PAP,
-             S_TOL = CG_TOL, S_MAXIT = CG_MAXIT };
+             S_TOL = CG_TOL, S_MAXIT = CG_MAXIT, S_IT0 = CG_IT0 };
 // partial-sum arrays inside `partial` (kRedBlocks each)
 enum : int { P_PAP = 0, P_RZ = 1, P_RR = 2, P_ROZ = 3, P_COUNT = 4 };
 
@@ -91,6 +91,8 @@ __global__ void k_init_fin(double *__restrict__ partial, double *__restrict__ sc
         scal[S_DONE0] = (maxiter <= 0) ? 1.0 : 0.0;
         scal[S_DONE1] = 0.0;
         scal[S_ITER] = 0.0;
+        scal[S_IT0] = 0.0;
+        scal[S_IT0 + 1] = 0.0;
         scal[S_TOL] = tol;
         scal[S_MAXIT] = (double)maxiter;
         scal[S_NRM] = 0.0;
@@ -149,7 +151,7 @@ k_step(int64_t n, double *__restrict__ y, double *__restrict__ r, const double *
 __global__ void __launch_bounds__(kThreads)
 k_finish(int64_t n, double *__restrict__ p, const double *__restrict__ r,
          const double *__restrict__ z, double *__restrict__ partial, double *__restrict__ scal,
-         int parity, int flexible, int host_iter)
+         int parity, int flexible)
 {
     __shared__ double sh[kThreads / 64];
     if (scal[S_DONE0 + parity] != 0.0) {
@@ -162,17 +164,19 @@ k_finish(int64_t n, double *__restrict__ p, const double *__restrict__ r,
     const double nrm = sqrt(rr);
     const double rzold = scal[S_RZ0 + parity];
     const bool conv = nrm < scal[S_TOL];
-    const int iter_next = host_iter + 1;
+    const int iter = (int)scal[S_IT0 + parity];
+    const int iter_next = iter + 1;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         scal[S_NRM] = nrm;
         if (conv) {
             scal[S_DONE0 + (parity ^ 1)] = 1.0;
-            scal[S_ITER] = (double)host_iter;
+            scal[S_ITER] = (double)iter;
         } else {
             scal[S_ITER] = (double)iter_next;
             scal[S_DONE0 + (parity ^ 1)] = (iter_next >= (int)scal[S_MAXIT]) ? 1.0 : 0.0;
             scal[S_RZ0 + (parity ^ 1)] = rz;
         }
+        scal[S_IT0 + (parity ^ 1)] = (double)iter_next;
     }
     if (conv) return;
     const double beta = (rz - (flexible ? roz : 0.0)) / rzold;
@@ -280,10 +284,10 @@ void cg_step(int64_t n, double *y, double *r, double *p, const double *Ap, const
 }
 
 void cg_finish_iter(int64_t n, double *r, double *p, const double *z, double *partial, double *scal,
-                    int parity, int flexible, int host_iter, hipStream_t st)
+                    int parity, int flexible, hipStream_t st)
 {
     hipLaunchKernelGGL(k_finish, dim3(kRedBlocks), dim3(kThreads), 0, st, n, p, r, z, partial, scal,
-                       parity, flexible, host_iter);
+                       parity, flexible);
 }
 
 void vec_axpby(int64_t n, double a, const double *x, double b, const double *y, double *out,

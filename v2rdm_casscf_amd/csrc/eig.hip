@@ -168,6 +168,19 @@ __device__ int pair_jacobi(double *S, double (&q)[kP / 4], double tol, int maxsw
     for (int sweep = 0; sweep < maxsweeps; sweep++) {
         if (threadIdx.x == 0) *sh_cnt = 0;
         __syncthreads();
+        // the circle positions of this thread's slots at step r, advanced by one
+        // (mod 63) per step instead of recomputed: slot 0 = (63, r), slot k > 0 =
+        // (r + k, r - k) mod 63
+        int ca[kUTper][2], cb[kUTper][2];
+#pragma unroll
+        for (int u = 0; u < kUTper; u++) {
+            const int sl[2] = {myP[u] < 0 ? 0 : myP[u], myR[u] < 0 ? 0 : myR[u]};
+#pragma unroll
+            for (int v = 0; v < 2; v++) {
+                ca[u][v] = sl[v] == 0 ? kP - 1 : sl[v];
+                cb[u][v] = sl[v] == 0 ? 0 : kP - 1 - sl[v];
+            }
+        }
         for (int r = 0; r < kP - 1; r++) {
             const int pl = circle_partner(r, lane);
             if (wave == 0) {
@@ -211,9 +224,8 @@ __device__ int pair_jacobi(double *S, double (&q)[kP / 4], double tol, int maxsw
                         cf[u][2] = sh_c[R];
                         cf[u][3] = sh_s[R];
                         act[u] = !(cf[u][1] == 0.0 && cf[u][3] == 0.0);
-                        int p0, p1, r0, r1;
-                        slot_pair(r, P, p0, p1);
-                        slot_pair(r, R, r0, r1);
+                        const int p0 = min(ca[u][0], cb[u][0]), p1 = max(ca[u][0], cb[u][0]);
+                        const int r0 = min(ca[u][1], cb[u][1]), r1 = max(ca[u][1], cb[u][1]);
                         auto ad = [](int i, int j) { return i < j ? i * kLd + j : j * kLd + i; };
                         addr[u][0] = ad(p0, r0);
                         addr[u][1] = ad(p0, r1);
@@ -243,17 +255,23 @@ __device__ int pair_jacobi(double *S, double (&q)[kP / 4], double tol, int maxsw
             // Q <- Q R in registers; every lane takes part in the shuffles
             {
                 const int k = circle_slot(r, lane);
-                const double c = sh_c[k], sn = sh_s[k];
-                const bool first = lane < pl;
+                const double c = sh_c[k], sn = lane < pl ? -sh_s[k] : sh_s[k];
                 // all shuffles first, then the arithmetic: one wait for the
                 // whole batch instead of one per row
                 double o[kP / 4];
 #pragma unroll
                 for (int i = 0; i < kP / 4; i++) o[i] = __shfl(q[i], pl, 64);
 #pragma unroll
-                for (int i = 0; i < kP / 4; i++)
-                    q[i] = first ? c * q[i] - sn * o[i] : sn * o[i] + c * q[i];
+                for (int i = 0; i < kP / 4; i++) q[i] = c * q[i] + sn * o[i];
             }
+            // advance the slot positions to step r + 1
+#pragma unroll
+            for (int u = 0; u < kUTper; u++)
+#pragma unroll
+                for (int v = 0; v < 2; v++) {
+                    if (ca[u][v] != kP - 1) ca[u][v] = ca[u][v] == kP - 2 ? 0 : ca[u][v] + 1;
+                    cb[u][v] = cb[u][v] == kP - 2 ? 0 : cb[u][v] + 1;
+                }
             __syncthreads();
         }
         const int cnt = *sh_cnt;

@@ -40,7 +40,9 @@ void launch_row_norm2(const MapPlan &p, double *d, hipStream_t st);
 // pairs indexed by the iteration parity
 enum CgSlot : int {
     CG_RZ0 = 16, CG_RZ1 = 17, CG_DONE0 = 18, CG_DONE1 = 19, CG_NRM = 20, CG_ITER = 21,
-    CG_ALPHA = 22, CG_BETA = 23, CG_PAP = 24, CG_TOL = 25, CG_MAXIT = 26, CG_NSLOTS = 32
+    CG_ALPHA = 22, CG_BETA = 23, CG_PAP = 24, CG_TOL = 25, CG_MAXIT = 26,
+    CG_IT0 = 27, CG_IT1 = 28,   // iteration counter, ping-pong like r.z
+    CG_NSLOTS = 32
 };
 constexpr int kRedBlocks = 2048;    // partial-sum slots per reduction (= grid of the vector kernels)
 constexpr int kPartialArrays = 4;   // partial buffer holds 4 * kRedBlocks doubles
@@ -54,8 +56,10 @@ void cg_dot_from_rows(int64_t ntiles, const double *rowpart, double *partial, co
                       int parity, hipStream_t st);
 void cg_step(int64_t n, double *y, double *r, double *p, const double *Ap, const double *minv,
              double *z, double *rprev, double *partial, double *scal, int parity, hipStream_t st);
+// the iteration number is read from the device counter of this parity, so a
+// chunk of iterations can be captured once and replayed
 void cg_finish_iter(int64_t n, double *r, double *p, const double *z, double *partial, double *scal,
-                    int parity, int flexible, int host_iter, hipStream_t st);
+                    int parity, int flexible, hipStream_t st);
 
 // ---- vector helpers (cg.hip) ----
 void vec_axpby(int64_t n, double a, const double *x, double b, const double *y, double *out,
