@@ -92,8 +92,16 @@ struct v2g_solver {
     DevBuf<double> Mw, Aw, Tw, Vw, Qbuf, tol, frop;
     DevBuf<int> active, counter;
     DevBuf<v2g::GemmDesc> d_gemm_warm1, d_gemm_warm2, d_gemm_rec;
-    int max_pairs = 0, max_nb = 0, max_N = 0;
-    int64_t max_tiles = 0;
+    int max_nb = 0, max_N = 0;
+    // block-Jacobi lanes: the large blocks are split into two groups of equal
+    // work (sum of N^3), each advancing its rounds on its own stream, so that
+    // one group's latency-bound pair problems overlap the other's tile updates.
+    // Lane l holds large[lane_off[l] .. lane_off[l] + lane_n[l]).
+    int nlanes = 1;
+    int lane_off[2] = {0, 0}, lane_n[2] = {0, 0}, lane_pairs[2] = {0, 0}, lane_nb[2] = {0, 0};
+    int64_t lane_tiles[2] = {0, 0};
+    hipStream_t st2 = nullptr;
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     bool warm = false;
     int last_sweeps = 0;
     v2g_iter_callback cb = nullptr;
@@ -121,6 +129,9 @@ struct v2g_solver {
     ~v2g_solver()
     {
         if (cg_exec) (void)hipGraphExecDestroy(cg_exec);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (st2) (void)hipStreamDestroy(st2);
         if (own) (void)hipStreamDestroy(own);
     }
     void ensure_jacobi();
@@ -263,12 +274,40 @@ v2g_solver::v2g_solver(const v2g_layout_desc &d)
         woff += (int64_t)e.N * e.N;
         qoff += (int64_t)np * 64 * 64;
         poff += np;
-        max_pairs = std::max(max_pairs, np);
         max_nb = std::max(max_nb, e.N / 32);
         max_N = std::max(max_N, e.N);
-        max_tiles = std::max(max_tiles, (int64_t)np * (np + 1) / 2 + (int64_t)np * np);
         large.push_back(e);
     }
+    // two lanes by longest-processing-time on N^3 (largest blocks first)
+    if (large.size() >= 2) {
+        std::vector<v2g::EigBlock> sorted = large, g[2];
+        std::stable_sort(sorted.begin(), sorted.end(),
+                         [](const v2g::EigBlock &x, const v2g::EigBlock &y) { return x.N > y.N; });
+        double load[2] = {0.0, 0.0};
+        for (const auto &e : sorted) {
+            const int l = load[0] <= load[1] ? 0 : 1;
+            g[l].push_back(e);
+            load[l] += (double)e.N * e.N * e.N;
+        }
+        large = g[0];
+        large.insert(large.end(), g[1].begin(), g[1].end());
+        nlanes = 2;
+        lane_n[0] = (int)g[0].size();
+        lane_n[1] = (int)g[1].size();
+        lane_off[1] = lane_n[0];
+        HIPCHK(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
+    } else {
+        lane_n[0] = (int)large.size();
+    }
+    for (int l = 0; l < nlanes; l++)
+        for (int i = lane_off[l]; i < lane_off[l] + lane_n[l]; i++) {
+            const int np = large[i].N / 64;
+            lane_pairs[l] = std::max(lane_pairs[l], np);
+            lane_nb[l] = std::max(lane_nb[l], large[i].N / 32);
+            lane_tiles[l] = std::max(lane_tiles[l], (int64_t)np * (np + 1) / 2 + (int64_t)np * np);
+        }
     d_small.upload(small);
     d_large.upload(large);
     if (!large.empty()) {
@@ -445,8 +484,8 @@ int v2g_solver::cg(const double *rhs, double *y, double tolv, int maxiter, int p
             v2g::launch_atu(plan, p.p, t.p, st, done);
             v2g::launch_au(plan, t.p, Ap.p, st, done, p.p, rowpart.p);   // Ap and p.Ap
             v2g::cg_dot_from_rows(plan.row_tiles, rowpart.p, partial.p, scal.p, par, st);
-            v2g::cg_step(n, y, r.p, p.p, Ap.p, minv, nullptr, nullptr, partial.p, scal.p, par, st);
-            v2g::cg_finish_iter(n, r.p, p.p, minv, partial.p, scal.p, par, flexible, st);
+            v2g::cg_step(n, r.p, Ap.p, minv, partial.p, scal.p, par, st);
+            v2g::cg_finish_iter(n, y, r.p, p.p, minv, partial.p, scal.p, par, flexible, st);
         }
     };
     hipGraphExec_t exec = nullptr;
@@ -503,7 +542,7 @@ void v2g_solver::update_xz(double mu, const double *y, const double *c, double *
     if (large.empty()) return;
     const int nl = (int)large.size();
     const int cold = warm ? 0 : 1;
-    v2g::eig_build_m(d_large.p, nl, ATy.p, c, x, mu, Mw.p, Aw.p, Vw.p, cold, st);
+    v2g::eig_build_m(d_large.p, nl, max_N, ATy.p, c, x, mu, Mw.p, Aw.p, Vw.p, cold, st);
     v2g::eig_tol(d_large.p, nl, Mw.p, tol.p, frop.p, st);
     if (!cold) {
         // A = V^T (M V)
@@ -527,8 +566,24 @@ void v2g_solver::update_xz(double mu, const double *y, const double *c, double *
     for (; sweeps < maxsweeps; sweeps++) {
         HIPCHK(hipMemsetAsync(counter.p, 0, sizeof(int), st));
         for (int rnd = 0; rnd < max_nb - 1; rnd++)
-            v2g::eig_round(d_large.p, nl, max_pairs, max_tiles, rnd, Aw.p, Vw.p, Qbuf.p, active.p,
-                           tol.p, counter.p, inner, st);
+            for (int l = 0; l < nlanes; l++) {
+                if (rnd >= lane_nb[l] - 1) continue;
+                const v2g::EigBlock *bl = d_large.p + lane_off[l];
+                hipStream_t s = l ? st2 : st;
+                v2g::eig_pairs(bl, lane_n[l], lane_pairs[l], rnd, Aw.p, Qbuf.p, active.p,
+                               tol.p + lane_off[l], counter.p, inner, s);
+                if (l == 0 && rnd == 0 && nlanes == 2) {
+                    // lane 1 starts once lane 0's first pair problems are done,
+                    // so each lane's tile updates run during the other's pairs
+                    HIPCHK(hipEventRecord(ev_fork, st));
+                    HIPCHK(hipStreamWaitEvent(st2, ev_fork, 0));
+                }
+                v2g::eig_apply(bl, lane_n[l], lane_tiles[l], rnd, Aw.p, Vw.p, Qbuf.p, active.p, s);
+            }
+        if (nlanes == 2) {
+            HIPCHK(hipEventRecord(ev_join, st2));
+            HIPCHK(hipStreamWaitEvent(st, ev_join, 0));
+        }
         int cnt = 0;
         HIPCHK(hipMemcpyAsync(&cnt, counter.p, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));

@@ -113,11 +113,12 @@ k_dot(int64_t n, const double *__restrict__ a, const double *__restrict__ b,
     if (threadIdx.x == 0) partial[P_PAP * kRedBlocks + blockIdx.x] = t;
 }
 
-// alpha = rz / pap; y += alpha p; r -= alpha Ap; z = M^-1 r; partial r.z, r.r, r_old.z
+// alpha = rz / pap; r -= alpha Ap; z = M^-1 r; partial r.z, r.r, r_old.z.
+// y += alpha p is left to k_finish, which reads p anyway (one vector pass less)
 __global__ void __launch_bounds__(kThreads)
-k_step(int64_t n, double *__restrict__ y, double *__restrict__ r, const double *__restrict__ p,
-       const double *__restrict__ Ap, const double *__restrict__ minv, double *__restrict__ z,
-       double *__restrict__ partial, double *__restrict__ scal, int parity)
+k_step(int64_t n, double *__restrict__ r, const double *__restrict__ Ap,
+       const double *__restrict__ minv, double *__restrict__ partial, double *__restrict__ scal,
+       int parity)
 {
     __shared__ double sh[kThreads / 64];
     if (scal[S_DONE0 + parity] != 0.0) return;
@@ -126,7 +127,6 @@ k_step(int64_t n, double *__restrict__ y, double *__restrict__ r, const double *
     double rz = 0.0, rr = 0.0, roz = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kThreads) {
-        y[i] += alpha * p[i];
         const double ro = r[i];
         const double rn = ro - alpha * Ap[i];
         r[i] = rn;
@@ -147,9 +147,10 @@ k_step(int64_t n, double *__restrict__ y, double *__restrict__ r, const double *
     }
 }
 
-// convergence test and p = M^-1 r + beta p (z carries M^-1 here, not z)
+// y += alpha p (alpha from k_step), convergence test and p = M^-1 r + beta p
+// (z carries M^-1 here, not z)
 __global__ void __launch_bounds__(kThreads)
-k_finish(int64_t n, double *__restrict__ p, const double *__restrict__ r,
+k_finish(int64_t n, double *__restrict__ y, double *__restrict__ p, const double *__restrict__ r,
          const double *__restrict__ z, double *__restrict__ partial, double *__restrict__ scal,
          int parity, int flexible)
 {
@@ -158,6 +159,7 @@ k_finish(int64_t n, double *__restrict__ p, const double *__restrict__ r,
         if (blockIdx.x == 0 && threadIdx.x == 0) scal[S_DONE0 + (parity ^ 1)] = 1.0;
         return;
     }
+    const double alpha = scal[S_ALPHA];
     const double rz = all_sum(partial + P_RZ * kRedBlocks, sh);
     const double rr = all_sum(partial + P_RR * kRedBlocks, sh);
     const double roz = flexible ? all_sum(partial + P_ROZ * kRedBlocks, sh) : 0.0;
@@ -166,6 +168,7 @@ k_finish(int64_t n, double *__restrict__ p, const double *__restrict__ r,
     const bool conv = nrm < scal[S_TOL];
     const int iter = (int)scal[S_IT0 + parity];
     const int iter_next = iter + 1;
+    // block 0 writes only next-parity slots and values no block of this launch reads
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         scal[S_NRM] = nrm;
         if (conv) {
@@ -178,12 +181,20 @@ k_finish(int64_t n, double *__restrict__ p, const double *__restrict__ r,
         }
         scal[S_IT0 + (parity ^ 1)] = (double)iter_next;
     }
-    if (conv) return;
+    if (conv) {
+        for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
+             i += (int64_t)gridDim.x * kThreads)
+            y[i] += alpha * p[i];
+        return;
+    }
     const double beta = (rz - (flexible ? roz : 0.0)) / rzold;
     if (blockIdx.x == 0 && threadIdx.x == 0) scal[S_BETA] = beta;
     for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kThreads)
-        p[i] = (z ? z[i] * r[i] : r[i]) + beta * p[i];
+         i += (int64_t)gridDim.x * kThreads) {
+        const double pi = p[i];
+        y[i] += alpha * pi;
+        p[i] = (z ? z[i] * r[i] : r[i]) + beta * pi;
+    }
 }
 
 // partial[P_PAP][b] = sum of a contiguous chunk of the per-tile partials
@@ -275,19 +286,18 @@ void cg_dot_from_rows(int64_t ntiles, const double *rowpart, double *partial, co
                        partial, scal, parity);
 }
 
-void cg_step(int64_t n, double *y, double *r, double *p, const double *Ap, const double *minv,
-             double *z, double *rprev, double *partial, double *scal, int parity, hipStream_t st)
+void cg_step(int64_t n, double *r, const double *Ap, const double *minv, double *partial,
+             double *scal, int parity, hipStream_t st)
 {
-    (void)rprev;
-    hipLaunchKernelGGL(k_step, dim3(kRedBlocks), dim3(kThreads), 0, st, n, y, r, p, Ap, minv, z,
-                       partial, scal, parity);
+    hipLaunchKernelGGL(k_step, dim3(kRedBlocks), dim3(kThreads), 0, st, n, r, Ap, minv, partial,
+                       scal, parity);
 }
 
-void cg_finish_iter(int64_t n, double *r, double *p, const double *z, double *partial, double *scal,
-                    int parity, int flexible, hipStream_t st)
+void cg_finish_iter(int64_t n, double *y, double *r, double *p, const double *z, double *partial,
+                    double *scal, int parity, int flexible, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_finish, dim3(kRedBlocks), dim3(kThreads), 0, st, n, p, r, z, partial, scal,
-                       parity, flexible);
+    hipLaunchKernelGGL(k_finish, dim3(kRedBlocks), dim3(kThreads), 0, st, n, y, p, r, z, partial,
+                       scal, parity, flexible);
 }
 
 void vec_axpby(int64_t n, double a, const double *x, double b, const double *y, double *out,

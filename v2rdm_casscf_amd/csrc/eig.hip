@@ -111,7 +111,8 @@ __device__ int lds_jacobi(double *S, double *Q, double tol, int maxsweeps, int *
 // The pair sub-problems of the large-block path use a leaner variant:
 //  * S keeps only its upper triangle (element (i, j) lives at (min, max)); a
 //    step updates the 528 pair blocks (P <= R) instead of all 1024;
-//  * Q lives in registers: wave w, lane l holds column l, rows 16w .. 16w+15.
+//  * Q lives in registers: wave w, lane l holds column l, rows
+//    kRowsPerWave w .. kRowsPerWave (w + 1) - 1.
 //    A column rotation needs the partner column only, which sits in the
 //    partner lane of the same wave, so Q moves through lane shuffles and never
 //    touches LDS.  That leaves one 64 x 65 FP64 array of LDS per workgroup.
@@ -149,9 +150,14 @@ __device__ __forceinline__ void slot_pair(int r, int k, int &p, int &q)
 }
 
 constexpr int kUT = (kP / 2) * (kP / 2 + 1) / 2;   // 528 pair blocks with P <= R
-constexpr int kUTper = (kUT + kT - 1) / kT;         // blocks per thread
+// threads of the pair-problem workgroup: few pair problems run at once (a few
+// hundred per round), so a step's latency, not throughput, sets the time; 16
+// waves give every thread one S block and 4 rows of Q per step
+constexpr int kTJ = 1024;
+constexpr int kRowsPerWave = kP * 64 / kTJ;         // rows of Q a wave holds
+constexpr int kUTper = (kUT + kTJ - 1) / kTJ;       // S blocks per thread
 
-__device__ int pair_jacobi(double *S, double (&q)[kP / 4], double tol, int maxsweeps,
+__device__ int pair_jacobi(double *S, double (&q)[kRowsPerWave], double tol, int maxsweeps,
                            const unsigned short *ut, int *sh_cnt, double *sh_c, double *sh_s,
                            int *sh_pa, int *sh_pb)
 {
@@ -160,7 +166,7 @@ __device__ int pair_jacobi(double *S, double (&q)[kP / 4], double tol, int maxsw
     int myP[kUTper], myR[kUTper];
 #pragma unroll
     for (int u = 0; u < kUTper; u++) {
-        const int w = threadIdx.x + u * kT;
+        const int w = threadIdx.x + u * kTJ;
         myP[u] = w < kUT ? (ut[w] & 0xff) : -1;
         myR[u] = w < kUT ? (ut[w] >> 8) : -1;
     }
@@ -258,11 +264,11 @@ __device__ int pair_jacobi(double *S, double (&q)[kP / 4], double tol, int maxsw
                 const double c = sh_c[k], sn = lane < pl ? -sh_s[k] : sh_s[k];
                 // all shuffles first, then the arithmetic: one wait for the
                 // whole batch instead of one per row
-                double o[kP / 4];
+                double o[kRowsPerWave];
 #pragma unroll
-                for (int i = 0; i < kP / 4; i++) o[i] = __shfl(q[i], pl, 64);
+                for (int i = 0; i < kRowsPerWave; i++) o[i] = __shfl(q[i], pl, 64);
 #pragma unroll
-                for (int i = 0; i < kP / 4; i++) q[i] = c * q[i] + sn * o[i];
+                for (int i = 0; i < kRowsPerWave; i++) q[i] = c * q[i] + sn * o[i];
             }
             // advance the slot positions to step r + 1
 #pragma unroll
@@ -331,26 +337,64 @@ k_small(const EigBlock *__restrict__ blocks, const double *__restrict__ aty,
 }
 
 // ---- large blocks -----------------------------------------------------------
+// M = sym(A^T y - c + mu x) into the padded N x N workspace, one 64 x 64 tile
+// pair (P, R), P <= R, per workgroup: both tiles are read with coalesced rows,
+// the transpose goes through LDS, and the (R, P) tile is written as the
+// transpose of the (P, R) one.  Cold calls also set A = M, V = I.
 __global__ void __launch_bounds__(kT)
 k_build_m(const EigBlock *__restrict__ blocks, const double *__restrict__ aty,
           const double *__restrict__ c, const double *__restrict__ x, double mu,
           double *__restrict__ Mw, double *__restrict__ Aw, double *__restrict__ Vw, int cold)
 {
+    __shared__ double S[kP * kLd];
     const EigBlock B = blocks[blockIdx.y];
-    const int64_t NN = (int64_t)B.N * B.N;
+    const int nt = B.N / kP;
+    int64_t rem = blockIdx.x;
+    if (rem >= (int64_t)nt * (nt + 1) / 2) return;
+    int P = 0;
+    while (rem >= nt - P) { rem -= nt - P; P++; }
+    const int R = P + (int)rem;
     const int n = B.n;
-    for (int64_t w = blockIdx.x * (int64_t)kT + threadIdx.x; w < NN; w += (int64_t)gridDim.x * kT) {
-        const int p = (int)(w / B.N), q = (int)(w % B.N);
-        double v = 0.0;
-        if (p < n && q < n) {
-            const int64_t ipq = B.xoff + (int64_t)p * n + q, iqp = B.xoff + (int64_t)q * n + p;
-            v = 0.5 * ((aty[ipq] - c[ipq] + mu * x[ipq]) + (aty[iqp] - c[iqp] + mu * x[iqp]));
-        }
-        Mw[B.woff + w] = v;
+    const int j = threadIdx.x & (kP - 1), i0 = threadIdx.x / kP;
+    constexpr int kU = kP * kP / kT;
+    auto g = [&](int p, int q) -> double {
+        if (p >= n || q >= n) return 0.0;
+        const int64_t i = B.xoff + (int64_t)p * n + q;
+        return aty[i] - c[i] + mu * x[i];
+    };
+    // (R, P) tile into LDS, (P, R) tile into registers
+    double v[kU];
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+        const int i = i0 + u * (kT / kP);
+        S[i * kLd + j] = g(R * kP + i, P * kP + j);
+        v[u] = g(P * kP + i, R * kP + j);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+        const int i = i0 + u * (kT / kP);
+        v[u] = 0.5 * (v[u] + S[j * kLd + i]);
+    }
+    auto put = [&](int p, int q, double val) {
+        const int64_t w = B.woff + (int64_t)p * B.N + q;
+        Mw[w] = val;
         if (cold) {
-            Aw[B.woff + w] = v;
-            Vw[B.woff + w] = (p == q) ? 1.0 : 0.0;
+            Aw[w] = val;
+            Vw[w] = (p == q) ? 1.0 : 0.0;
         }
+    };
+#pragma unroll
+    for (int u = 0; u < kU; u++) put(P * kP + i0 + u * (kT / kP), R * kP + j, v[u]);
+    if (P == R) return;
+    __syncthreads();   // every thread has read S
+#pragma unroll
+    for (int u = 0; u < kU; u++) S[(i0 + u * (kT / kP)) * kLd + j] = v[u];
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < kU; u++) {
+        const int i = i0 + u * (kT / kP);
+        put(R * kP + i, P * kP + j, S[j * kLd + i]);
     }
 }
 
@@ -395,7 +439,7 @@ __device__ __forceinline__ int pair_index(int a, int b, int i)
     return i < 32 ? a * 32 + i : b * 32 + (i - 32);
 }
 
-__global__ void __launch_bounds__(kT)
+__global__ void __launch_bounds__(kTJ)
 k_pair_eig(const EigBlock *__restrict__ blocks, int round, const double *__restrict__ Aw,
            double *__restrict__ Qbuf, int *__restrict__ active, const double *__restrict__ tol,
            int *__restrict__ counter, int inner_sweeps)
@@ -413,18 +457,28 @@ k_pair_eig(const EigBlock *__restrict__ blocks, int round, const double *__restr
     const double *A = Aw + B.woff;
     const double tl = tol[blockIdx.y];
     if (threadIdx.x == 0) sh_cnt = 0;
-    for (int w = threadIdx.x; w < kUT; w += kT) {
+    for (int w = threadIdx.x; w < kUT; w += kTJ) {
         int P = 0, rem = w;
         while (rem >= kP / 2 - P) { rem -= kP / 2 - P; P++; }
         ut[w] = (unsigned short)(P | ((P + rem) << 8));
     }
     __syncthreads();
     int big = 0;
-    for (int w = threadIdx.x; w < kP * kP; w += kT) {
-        const int i = w / kP, j = w % kP;
-        const double v = A[(int64_t)pair_index(a, b, i) * B.N + pair_index(a, b, j)];
-        S[i * kLd + j] = v;
-        big |= (i != j && fabs(v) > tl);
+    {
+        // thread (i0, j) loads rows i0, i0 + kTJ / kP, ... of column j, all
+        // loads in flight at once
+        const int j = threadIdx.x & (kP - 1), i0 = threadIdx.x / kP;
+        const int64_t colj = pair_index(a, b, j);
+        double v[kP * kP / kTJ];
+#pragma unroll
+        for (int u = 0; u < kP * kP / kTJ; u++)
+            v[u] = A[(int64_t)pair_index(a, b, i0 + u * (kTJ / kP)) * B.N + colj];
+#pragma unroll
+        for (int u = 0; u < kP * kP / kTJ; u++) {
+            const int i = i0 + u * (kTJ / kP);
+            S[i * kLd + j] = v[u];
+            big |= (i != j && fabs(v[u]) > tl);
+        }
     }
     if (big) atomicOr(&sh_cnt, 1);
     __syncthreads();
@@ -435,13 +489,13 @@ k_pair_eig(const EigBlock *__restrict__ blocks, int round, const double *__restr
         return;
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    double q[kP / 4];
+    double q[kRowsPerWave];
 #pragma unroll
-    for (int i = 0; i < kP / 4; i++) q[i] = (wave * (kP / 4) + i == lane) ? 1.0 : 0.0;
+    for (int i = 0; i < kRowsPerWave; i++) q[i] = (wave * kRowsPerWave + i == lane) ? 1.0 : 0.0;
     const int rot = pair_jacobi(S, q, tl, inner_sweeps, ut, &sh_cnt, sh_c, sh_s, sh_pa, sh_pb);
     double *Qo = Qbuf + B.qoff + (int64_t)k * kP * kP;
 #pragma unroll
-    for (int i = 0; i < kP / 4; i++) Qo[(wave * (kP / 4) + i) * kP + lane] = q[i];
+    for (int i = 0; i < kRowsPerWave; i++) Qo[(wave * kRowsPerWave + i) * kP + lane] = q[i];
     if (threadIdx.x == 0) {
         active[B.poff + k] = rot > 0;
         if (rot > 0) atomicAdd(counter, 1);
@@ -523,9 +577,16 @@ k_apply(const EigBlock *__restrict__ blocks, int round, double *__restrict__ Aw,
     circle_pair(nbk, round, R, ra, rb);
     double *M = isA ? Aw + B.woff : Vw + B.woff;
     auto row_of = [&](int i) -> int64_t { return isA ? pair_index(pa, pb, i) : rt * kP + i; };
-    for (int w = threadIdx.x; w < kP * kP; w += kT) {
-        const int i = w / kP, j = w % kP;
-        T[i * kLd + j] = M[row_of(i) * B.N + pair_index(ra, rb, j)];
+    // thread (i0, j) moves rows i0, i0 + 4, ... of column j: all 16 loads are
+    // issued before the first LDS write, so each thread keeps 16 in flight
+    const int j = threadIdx.x & (kP - 1), i0 = threadIdx.x / kP;
+    const int64_t colj = pair_index(ra, rb, j);
+    {
+        double v[kP * kP / kT];
+#pragma unroll
+        for (int u = 0; u < kP * kP / kT; u++) v[u] = M[row_of(i0 + u * (kT / kP)) * B.N + colj];
+#pragma unroll
+        for (int u = 0; u < kP * kP / kT; u++) T[(i0 + u * (kT / kP)) * kLd + j] = v[u];
     }
     __syncthreads();
     v4d acc[2][2];
@@ -543,15 +604,19 @@ k_apply(const EigBlock *__restrict__ blocks, int round, double *__restrict__ Aw,
         store_tile(T, acc);
     }
     __syncthreads();
-    for (int w = threadIdx.x; w < kP * kP; w += kT) {
-        const int i = w / kP, j = w % kP;
-        M[row_of(i) * B.N + pair_index(ra, rb, j)] = T[i * kLd + j];
+#pragma unroll
+    for (int u = 0; u < kP * kP / kT; u++) {
+        const int i = i0 + u * (kT / kP);
+        M[row_of(i) * B.N + colj] = T[i * kLd + j];
     }
     if (isA && P != R) {
-        // (R, P) tile = transpose
-        for (int w = threadIdx.x; w < kP * kP; w += kT) {
-            const int j = w / kP, i = w % kP;
-            M[(int64_t)pair_index(ra, rb, j) * B.N + pair_index(pa, pb, i)] = T[i * kLd + j];
+        // (R, P) tile = transpose: thread (i0, j) writes row pair_index(ra, rb, i)
+        // of the (R, P) tile at column pair_index(pa, pb, j)
+        const int64_t colt = pair_index(pa, pb, j);
+#pragma unroll
+        for (int u = 0; u < kP * kP / kT; u++) {
+            const int i = i0 + u * (kT / kP);
+            M[(int64_t)pair_index(ra, rb, i) * B.N + colt] = T[j * kLd + i];
         }
     }
 }
@@ -592,12 +657,13 @@ void eig_small(const EigBlock *blocks, int nblocks, const double *aty, const dou
         hipLaunchKernelGGL(k_small, dim3(nblocks), dim3(kT), 0, st, blocks, aty, c, x, z, mu);
 }
 
-void eig_build_m(const EigBlock *blocks, int nblocks, const double *aty, const double *c,
+void eig_build_m(const EigBlock *blocks, int nblocks, int max_N, const double *aty, const double *c,
                  const double *x, double mu, double *Mw, double *Aw, double *Vw, int cold,
                  hipStream_t st)
 {
-    hipLaunchKernelGGL(k_build_m, dim3(256, nblocks), dim3(kT), 0, st, blocks, aty, c, x, mu, Mw,
-                       Aw, Vw, cold);
+    const int nt = max_N / kP;
+    hipLaunchKernelGGL(k_build_m, dim3(nt * (nt + 1) / 2, nblocks), dim3(kT), 0, st, blocks, aty, c,
+                       x, mu, Mw, Aw, Vw, cold);
 }
 
 void eig_tol(const EigBlock *blocks, int nblocks, const double *Mw, double *tol, double *part,
@@ -607,12 +673,17 @@ void eig_tol(const EigBlock *blocks, int nblocks, const double *Mw, double *tol,
     hipLaunchKernelGGL(k_fro_fin, dim3((nblocks + 63) / 64), dim3(64), 0, st, nblocks, part, tol);
 }
 
-void eig_round(const EigBlock *blocks, int nblocks, int maxpairs, int64_t maxtiles, int round,
-               double *Aw, double *Vw, double *Qbuf, int *active, const double *tol, int *counter,
-               int inner_sweeps, hipStream_t st)
+void eig_pairs(const EigBlock *blocks, int nblocks, int maxpairs, int round, const double *Aw,
+               double *Qbuf, int *active, const double *tol, int *counter, int inner_sweeps,
+               hipStream_t st)
 {
-    hipLaunchKernelGGL(k_pair_eig, dim3(maxpairs, nblocks), dim3(kT), 0, st, blocks, round, Aw, Qbuf,
+    hipLaunchKernelGGL(k_pair_eig, dim3(maxpairs, nblocks), dim3(kTJ), 0, st, blocks, round, Aw, Qbuf,
                        active, tol, counter, inner_sweeps);
+}
+
+void eig_apply(const EigBlock *blocks, int nblocks, int64_t maxtiles, int round, double *Aw,
+               double *Vw, const double *Qbuf, const int *active, hipStream_t st)
+{
     hipLaunchKernelGGL(k_apply, dim3((unsigned)maxtiles, nblocks), dim3(kT), 0, st, blocks, round, Aw,
                        Vw, Qbuf, active);
 }

@@ -28,16 +28,20 @@ struct GemmDesc {
 
 void eig_small(const EigBlock *blocks, int nblocks, const double *aty, const double *c, double *x,
                double *z, double mu, hipStream_t st);
-void eig_build_m(const EigBlock *blocks, int nblocks, const double *aty, const double *c,
+// max_N: the largest padded dimension among the blocks
+void eig_build_m(const EigBlock *blocks, int nblocks, int max_N, const double *aty, const double *c,
                  const double *x, double mu, double *Mw, double *Aw, double *Vw, int cold,
                  hipStream_t st);
 // tol[b] = 2e-14 ||M_b||_F; part holds 64 * nblocks doubles
 void eig_tol(const EigBlock *blocks, int nblocks, const double *Mw, double *tol, double *part,
              hipStream_t st);
-// inner_sweeps: cap on the cyclic-Jacobi sweeps of each 64 x 64 pair sub-problem
-void eig_round(const EigBlock *blocks, int nblocks, int maxpairs, int64_t maxtiles, int round,
-               double *Aw, double *Vw, double *Qbuf, int *active, const double *tol, int *counter,
-               int inner_sweeps, hipStream_t st);
+// one block-Jacobi round: the pair sub-problems (inner_sweeps caps the cyclic
+// sweeps of each 64 x 64 pair), then the rotation of the A and V tiles
+void eig_pairs(const EigBlock *blocks, int nblocks, int maxpairs, int round, const double *Aw,
+               double *Qbuf, int *active, const double *tol, int *counter, int inner_sweeps,
+               hipStream_t st);
+void eig_apply(const EigBlock *blocks, int nblocks, int64_t maxtiles, int round, double *Aw,
+               double *Vw, const double *Qbuf, const int *active, hipStream_t st);
 void eig_scale(const EigBlock *blocks, int nblocks, const double *Aw, const double *Vw, double *Tw,
                double mu, hipStream_t st);
 void eig_z(const EigBlock *blocks, int nblocks, const double *x, const double *Mw, double *z,

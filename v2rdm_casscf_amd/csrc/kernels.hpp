@@ -54,12 +54,14 @@ void cg_dot(int64_t n, const double *a, const double *b, double *partial, double
 // p.Ap from the per-tile partials of the fused A.x launch
 void cg_dot_from_rows(int64_t ntiles, const double *rowpart, double *partial, const double *scal,
                       int parity, hipStream_t st);
-void cg_step(int64_t n, double *y, double *r, double *p, const double *Ap, const double *minv,
-             double *z, double *rprev, double *partial, double *scal, int parity, hipStream_t st);
-// the iteration number is read from the device counter of this parity, so a
-// chunk of iterations can be captured once and replayed
-void cg_finish_iter(int64_t n, double *r, double *p, const double *z, double *partial, double *scal,
-                    int parity, int flexible, hipStream_t st);
+// r -= alpha Ap and the partial dot products of the new residual
+void cg_step(int64_t n, double *r, const double *Ap, const double *minv, double *partial,
+             double *scal, int parity, hipStream_t st);
+// y += alpha p, convergence test, p update.  The iteration number is read from
+// the device counter of this parity, so a chunk of iterations can be captured
+// once and replayed
+void cg_finish_iter(int64_t n, double *y, double *r, double *p, const double *z, double *partial,
+                    double *scal, int parity, int flexible, hipStream_t st);
 
 // ---- vector helpers (cg.hip) ----
 void vec_axpby(int64_t n, double a, const double *x, double b, const double *y, double *out,
