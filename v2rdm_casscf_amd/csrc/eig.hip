@@ -41,28 +41,54 @@ __device__ __forceinline__ void circle_pair(int m, int r, int k, int &a, int &b)
     b = (r - k + mm) % mm;
 }
 
-// parallel cyclic Jacobi on the kP x kP symmetric matrix S (pitch kLd) in LDS,
-// accumulating rotations into Q.  Returns the number of rotations applied.
-__device__ int lds_jacobi(double *S, double *Q, double tol, int maxsweeps, int *sh_cnt,
+// Jacobi rotation (c, s) that annihilates apq.  t = tan(theta), the smaller
+// root: the textbook t = sgn(th) / (|th| + sqrt(th^2 + 1)), th = d / e with
+// d = aqq - app, e = 2 apq, multiplied through by |e|, gives
+// t = sgn(d) e / (|d| + sqrt(d^2 + e^2)): one division instead of three.
+// c = 1 / sqrt(1 + t^2) (1 <= 1 + t^2 <= 2) from the hardware reciprocal square
+// root and two Newton steps; s = t c.
+__device__ __forceinline__ void jacobi_rotation(double app, double aqq, double apq, double &c,
+                                                double &s)
+{
+    const double d = aqq - app, e = 2.0 * apq;
+    const double r = sqrt(fma(d, d, e * e));
+    double t;
+    if (r > 0.0) {
+        t = (d >= 0.0 ? e : -e) / (fabs(d) + r);
+    } else {   // d^2 + e^2 underflows: the textbook form
+        const double th = d / e;
+        t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+    }
+    const double x = fma(t, t, 1.0), hx = 0.5 * x;
+    double y = __builtin_amdgcn_rsq(x);
+    y = fma(y, fma(-hx * y, y, 0.5), y);
+    y = fma(y, fma(-hx * y, y, 0.5), y);
+    c = y;
+    s = t * y;
+}
+
+// parallel cyclic Jacobi on the leading m x m part (m even, m <= kP) of the
+// symmetric matrix S (pitch kLd) in LDS, accumulating rotations into Q; rows
+// and columns beyond m are zero in S and the identity in Q, and stay so.
+// Returns the number of rotations applied.
+__device__ int lds_jacobi(double *S, double *Q, int m, double tol, int maxsweeps, int *sh_cnt,
                           double *sh_c, double *sh_s, int *sh_pa, int *sh_pb)
 {
     int total = 0;
+    const int h = m / 2;
     for (int sweep = 0; sweep < maxsweeps; sweep++) {
         if (threadIdx.x == 0) *sh_cnt = 0;
         __syncthreads();
-        for (int r = 0; r < kP - 1; r++) {
-            if (threadIdx.x < kP / 2) {
+        for (int r = 0; r < m - 1; r++) {
+            if (threadIdx.x < h) {
                 int a, b;
-                circle_pair(kP, r, threadIdx.x, a, b);
+                circle_pair(m, r, threadIdx.x, a, b);
                 const int p = a < b ? a : b, q = a < b ? b : a;
                 const double apq = S[p * kLd + q];
                 double c = 1.0, s = 0.0;
                 if (fabs(apq) > tol) {
                     const double app = S[p * kLd + p], aqq = S[q * kLd + q];
-                    const double th = (aqq - app) / (2.0 * apq);
-                    const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
-                    c = 1.0 / sqrt(t * t + 1.0);
-                    s = t * c;
+                    jacobi_rotation(app, aqq, apq, c, s);
                     atomicAdd(sh_cnt, 1);
                 }
                 sh_c[threadIdx.x] = c;
@@ -72,8 +98,8 @@ __device__ int lds_jacobi(double *S, double *Q, double tol, int maxsweeps, int *
             }
             __syncthreads();
             // S <- R^T S R on every 2x2 (row pair, column pair) block
-            for (int w = threadIdx.x; w < (kP / 2) * (kP / 2); w += kT) {
-                const int P = w / (kP / 2), R = w % (kP / 2);
+            for (int w = threadIdx.x; w < h * h; w += kT) {
+                const int P = w / h, R = w % h;
                 const double cp = sh_c[P], sp = sh_s[P], cr = sh_c[R], sr = sh_s[R];
                 if (sp == 0.0 && sr == 0.0) continue;
                 const int p0 = sh_pa[P], p1 = sh_pb[P], r0 = sh_pa[R], r1 = sh_pb[R];
@@ -88,9 +114,9 @@ __device__ int lds_jacobi(double *S, double *Q, double tol, int maxsweeps, int *
                 S[p1 * kLd + r0] = cr * u10 - sr * u11;
                 S[p1 * kLd + r1] = sr * u10 + cr * u11;
             }
-            // Q <- Q R on column pairs
-            for (int w = threadIdx.x; w < kP * (kP / 2); w += kT) {
-                const int row = w / (kP / 2), R = w % (kP / 2);
+            // Q <- Q R on column pairs (rows beyond m are zero in these columns)
+            for (int w = threadIdx.x; w < m * h; w += kT) {
+                const int row = w / h, R = w % h;
                 const double cr = sh_c[R], sr = sh_s[R];
                 if (sr == 0.0) continue;
                 const int r0 = sh_pa[R], r1 = sh_pb[R];
@@ -197,10 +223,7 @@ __device__ int pair_jacobi(double *S, double (&q)[kRowsPerWave], double tol, int
                     double c = 1.0, sn = 0.0;
                     if (fabs(apq) > tol) {
                         const double app = S[p * kLd + p], aqq = S[qq * kLd + qq];
-                        const double th = (aqq - app) / (2.0 * apq);
-                        const double t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
-                        c = 1.0 / sqrt(t * t + 1.0);
-                        sn = t * c;
+                        jacobi_rotation(app, aqq, apq, c, sn);
                         rotated = true;
                     }
                     const int k = circle_slot(r, lane);
@@ -321,7 +344,9 @@ k_small(const EigBlock *__restrict__ blocks, const double *__restrict__ aty,
     double tot = 0.0;
     for (int i = 0; i < kT / 64; i++) tot += red[i];
     const double tol = 1e-15 * sqrt(tot) + 1e-300;
-    lds_jacobi(S, Q, tol, 60, &sh_cnt, sh_c, sh_s, sh_pa, sh_pb);
+    // the circle runs over n rounded up to even indices, not all kP
+    const int m = (n + 1) & ~1;
+    lds_jacobi(S, Q, m, tol, 60, &sh_cnt, sh_c, sh_s, sh_pa, sh_pb);
     if (threadIdx.x < kP) {
         const double l = S[threadIdx.x * kLd + threadIdx.x];
         lam[threadIdx.x] = l > 0.0 ? l / mu : 0.0;
@@ -330,7 +355,7 @@ k_small(const EigBlock *__restrict__ blocks, const double *__restrict__ aty,
     for (int w = threadIdx.x; w < n * n; w += kT) {
         const int p = w / n, q = w % n;
         double s = 0.0;
-        for (int j = 0; j < kP; j++) s += Q[p * kLd + j] * lam[j] * Q[q * kLd + j];
+        for (int j = 0; j < m; j++) s += Q[p * kLd + j] * lam[j] * Q[q * kLd + j];
         x[o + w] = s;
         z[o + w] = mu * s - Mv[p * kLd + q];
     }
