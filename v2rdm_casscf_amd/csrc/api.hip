@@ -93,15 +93,24 @@ struct v2g_solver {
     DevBuf<int> active, counter;
     DevBuf<v2g::GemmDesc> d_gemm_warm1, d_gemm_warm2, d_gemm_rec;
     int max_nb = 0, max_N = 0;
-    // block-Jacobi lanes: the large blocks are split into two groups of equal
-    // work (sum of N^3), each advancing its rounds on its own stream, so that
-    // one group's latency-bound pair problems overlap the other's tile updates.
-    // Lane l holds large[lane_off[l] .. lane_off[l] + lane_n[l]).
+    // block-Jacobi lanes: the large blocks are split into up to kMaxLanes groups
+    // of equal work (sum of N^3), each advancing its rounds on its own stream,
+    // so that latency-bound pair problems of some lanes overlap the
+    // bandwidth-bound tile updates of others.  Lane l holds
+    // large[lane_off[l] .. lane_off[l] + lane_n[l]); lane 0 runs on the engine
+    // stream, lane l > 0 on lane_st[l].  Two lanes: HIP maps a process's
+    // streams onto a few hardware queues, and with four lanes two of them
+    // shared a queue and ran one after the other (measured at H50).
+    static constexpr int kMaxLanes = 2;
+    // largest padded block from which a problem is treated as latency-bound:
+    // lanes and 1024-thread pair workgroups
+    static constexpr int kLaneMinN = 512;
     int nlanes = 1;
-    int lane_off[2] = {0, 0}, lane_n[2] = {0, 0}, lane_pairs[2] = {0, 0}, lane_nb[2] = {0, 0};
-    int64_t lane_tiles[2] = {0, 0};
-    hipStream_t st2 = nullptr;
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    int lane_off[kMaxLanes] = {}, lane_n[kMaxLanes] = {}, lane_pairs[kMaxLanes] = {},
+        lane_nb[kMaxLanes] = {};
+    int64_t lane_tiles[kMaxLanes] = {};
+    hipStream_t lane_st[kMaxLanes] = {};
+    hipEvent_t ev_fork = nullptr, ev_start[kMaxLanes] = {}, ev_join[kMaxLanes] = {};
     bool warm = false;
     int last_sweeps = 0;
     v2g_iter_callback cb = nullptr;
@@ -130,8 +139,11 @@ struct v2g_solver {
     {
         if (cg_exec) (void)hipGraphExecDestroy(cg_exec);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
-        if (ev_join) (void)hipEventDestroy(ev_join);
-        if (st2) (void)hipStreamDestroy(st2);
+        for (int l = 0; l < kMaxLanes; l++) {
+            if (ev_start[l]) (void)hipEventDestroy(ev_start[l]);
+            if (ev_join[l]) (void)hipEventDestroy(ev_join[l]);
+            if (lane_st[l]) (void)hipStreamDestroy(lane_st[l]);
+        }
         if (own) (void)hipStreamDestroy(own);
     }
     void ensure_jacobi();
@@ -278,26 +290,32 @@ v2g_solver::v2g_solver(const v2g_layout_desc &d)
         max_N = std::max(max_N, e.N);
         large.push_back(e);
     }
-    // two lanes by longest-processing-time on N^3 (largest blocks first)
-    if (large.size() >= 2) {
-        std::vector<v2g::EigBlock> sorted = large, g[2];
+    // lanes by longest-processing-time on N^3 (largest blocks first), for
+    // problems whose rounds are long enough (N >= kLaneMinN) to pay for the
+    // extra launches
+    if (large.size() >= 2 && max_N >= kLaneMinN) {
+        nlanes = std::min<int>(kMaxLanes, (int)large.size());
+        std::vector<v2g::EigBlock> sorted = large, g[kMaxLanes];
         std::stable_sort(sorted.begin(), sorted.end(),
                          [](const v2g::EigBlock &x, const v2g::EigBlock &y) { return x.N > y.N; });
-        double load[2] = {0.0, 0.0};
+        double load[kMaxLanes] = {};
         for (const auto &e : sorted) {
-            const int l = load[0] <= load[1] ? 0 : 1;
+            int l = 0;
+            for (int k = 1; k < nlanes; k++)
+                if (load[k] < load[l]) l = k;
             g[l].push_back(e);
             load[l] += (double)e.N * e.N * e.N;
         }
-        large = g[0];
-        large.insert(large.end(), g[1].begin(), g[1].end());
-        nlanes = 2;
-        lane_n[0] = (int)g[0].size();
-        lane_n[1] = (int)g[1].size();
-        lane_off[1] = lane_n[0];
-        HIPCHK(hipStreamCreateWithFlags(&st2, hipStreamNonBlocking));
+        large.clear();
+        for (int l = 0; l < nlanes; l++) {
+            lane_off[l] = (int)large.size();
+            lane_n[l] = (int)g[l].size();
+            large.insert(large.end(), g[l].begin(), g[l].end());
+            if (l > 0) HIPCHK(hipStreamCreateWithFlags(&lane_st[l], hipStreamNonBlocking));
+            HIPCHK(hipEventCreateWithFlags(&ev_start[l], hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&ev_join[l], hipEventDisableTiming));
+        }
         HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
-        HIPCHK(hipEventCreateWithFlags(&ev_join, hipEventDisableTiming));
     } else {
         lane_n[0] = (int)large.size();
     }
@@ -565,24 +583,29 @@ void v2g_solver::update_xz(double mu, const double *y, const double *c, double *
     int sweeps = 0;
     for (; sweeps < maxsweeps; sweeps++) {
         HIPCHK(hipMemsetAsync(counter.p, 0, sizeof(int), st));
+        if (nlanes > 1) {
+            HIPCHK(hipEventRecord(ev_fork, st));
+            for (int l = 2; l < nlanes; l += 2) HIPCHK(hipStreamWaitEvent(lane_st[l], ev_fork, 0));
+        }
         for (int rnd = 0; rnd < max_nb - 1; rnd++)
             for (int l = 0; l < nlanes; l++) {
                 if (rnd >= lane_nb[l] - 1) continue;
                 const v2g::EigBlock *bl = d_large.p + lane_off[l];
-                hipStream_t s = l ? st2 : st;
+                hipStream_t s = l ? lane_st[l] : st;
                 v2g::eig_pairs(bl, lane_n[l], lane_pairs[l], rnd, Aw.p, Qbuf.p, active.p,
-                               tol.p + lane_off[l], counter.p, inner, s);
-                if (l == 0 && rnd == 0 && nlanes == 2) {
-                    // lane 1 starts once lane 0's first pair problems are done,
-                    // so each lane's tile updates run during the other's pairs
-                    HIPCHK(hipEventRecord(ev_fork, st));
-                    HIPCHK(hipStreamWaitEvent(st2, ev_fork, 0));
+                               tol.p + lane_off[l], counter.p, inner, max_N >= kLaneMinN, s);
+                if (rnd == 0 && l % 2 == 0 && l + 1 < nlanes) {
+                    // lane l + 1 starts once lane l's first pair problems are
+                    // done, so in steady state each of the two runs its tile
+                    // updates while the other solves its pair problems
+                    HIPCHK(hipEventRecord(ev_start[l], s));
+                    HIPCHK(hipStreamWaitEvent(lane_st[l + 1], ev_start[l], 0));
                 }
                 v2g::eig_apply(bl, lane_n[l], lane_tiles[l], rnd, Aw.p, Vw.p, Qbuf.p, active.p, s);
             }
-        if (nlanes == 2) {
-            HIPCHK(hipEventRecord(ev_join, st2));
-            HIPCHK(hipStreamWaitEvent(st, ev_join, 0));
+        for (int l = 1; l < nlanes; l++) {
+            HIPCHK(hipEventRecord(ev_join[l], lane_st[l]));
+            HIPCHK(hipStreamWaitEvent(st, ev_join[l], 0));
         }
         int cnt = 0;
         HIPCHK(hipMemcpyAsync(&cnt, counter.p, sizeof(int), hipMemcpyDeviceToHost, st));

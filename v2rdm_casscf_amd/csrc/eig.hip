@@ -176,23 +176,28 @@ __device__ __forceinline__ void slot_pair(int r, int k, int &p, int &q)
 }
 
 constexpr int kUT = (kP / 2) * (kP / 2 + 1) / 2;   // 528 pair blocks with P <= R
-// threads of the pair-problem workgroup: few pair problems run at once (a few
-// hundred per round), so a step's latency, not throughput, sets the time; 16
-// waves give every thread one S block and 4 rows of Q per step
-constexpr int kTJ = 1024;
-constexpr int kRowsPerWave = kP * 64 / kTJ;         // rows of Q a wave holds
-constexpr int kUTper = (kUT + kTJ - 1) / kTJ;       // S blocks per thread
+// Workgroup size NT of the pair problems.  Large problems have a few hundred
+// pair problems per round, so a step's latency sets the time: NT = 1024 (16
+// waves, one S block and 4 rows of Q per thread) shortens it.  Many small
+// problems at once (bond scans) are throughput-bound: NT = 256 does the same
+// work in fewer instructions.
+template <int NT> struct PairShape {
+    static constexpr int kRowsPerWave = kP * 64 / NT;   // rows of Q a wave holds
+    static constexpr int kUTper = (kUT + NT - 1) / NT;  // S blocks per thread
+};
 
-__device__ int pair_jacobi(double *S, double (&q)[kRowsPerWave], double tol, int maxsweeps,
-                           const unsigned short *ut, int *sh_cnt, double *sh_c, double *sh_s,
-                           int *sh_pa, int *sh_pb)
+template <int NT>
+__device__ int pair_jacobi(double *S, double (&q)[PairShape<NT>::kRowsPerWave], double tol,
+                           int maxsweeps, const unsigned short *ut, int *sh_cnt, double *sh_c,
+                           double *sh_s, int *sh_pa, int *sh_pb)
 {
+    constexpr int kRowsPerWave = PairShape<NT>::kRowsPerWave, kUTper = PairShape<NT>::kUTper;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     // this thread's upper-triangle pair blocks (fixed for the whole solve)
     int myP[kUTper], myR[kUTper];
 #pragma unroll
     for (int u = 0; u < kUTper; u++) {
-        const int w = threadIdx.x + u * kTJ;
+        const int w = threadIdx.x + u * NT;
         myP[u] = w < kUT ? (ut[w] & 0xff) : -1;
         myR[u] = w < kUT ? (ut[w] >> 8) : -1;
     }
@@ -464,7 +469,8 @@ __device__ __forceinline__ int pair_index(int a, int b, int i)
     return i < 32 ? a * 32 + i : b * 32 + (i - 32);
 }
 
-__global__ void __launch_bounds__(kTJ)
+template <int NT>
+__global__ void __launch_bounds__(NT)
 k_pair_eig(const EigBlock *__restrict__ blocks, int round, const double *__restrict__ Aw,
            double *__restrict__ Qbuf, int *__restrict__ active, const double *__restrict__ tol,
            int *__restrict__ counter, int inner_sweeps)
@@ -482,7 +488,7 @@ k_pair_eig(const EigBlock *__restrict__ blocks, int round, const double *__restr
     const double *A = Aw + B.woff;
     const double tl = tol[blockIdx.y];
     if (threadIdx.x == 0) sh_cnt = 0;
-    for (int w = threadIdx.x; w < kUT; w += kTJ) {
+    for (int w = threadIdx.x; w < kUT; w += NT) {
         int P = 0, rem = w;
         while (rem >= kP / 2 - P) { rem -= kP / 2 - P; P++; }
         ut[w] = (unsigned short)(P | ((P + rem) << 8));
@@ -490,17 +496,17 @@ k_pair_eig(const EigBlock *__restrict__ blocks, int round, const double *__restr
     __syncthreads();
     int big = 0;
     {
-        // thread (i0, j) loads rows i0, i0 + kTJ / kP, ... of column j, all
+        // thread (i0, j) loads rows i0, i0 + NT / kP, ... of column j, all
         // loads in flight at once
         const int j = threadIdx.x & (kP - 1), i0 = threadIdx.x / kP;
         const int64_t colj = pair_index(a, b, j);
-        double v[kP * kP / kTJ];
+        double v[kP * kP / NT];
 #pragma unroll
-        for (int u = 0; u < kP * kP / kTJ; u++)
-            v[u] = A[(int64_t)pair_index(a, b, i0 + u * (kTJ / kP)) * B.N + colj];
+        for (int u = 0; u < kP * kP / NT; u++)
+            v[u] = A[(int64_t)pair_index(a, b, i0 + u * (NT / kP)) * B.N + colj];
 #pragma unroll
-        for (int u = 0; u < kP * kP / kTJ; u++) {
-            const int i = i0 + u * (kTJ / kP);
+        for (int u = 0; u < kP * kP / NT; u++) {
+            const int i = i0 + u * (NT / kP);
             S[i * kLd + j] = v[u];
             big |= (i != j && fabs(v[u]) > tl);
         }
@@ -514,10 +520,11 @@ k_pair_eig(const EigBlock *__restrict__ blocks, int round, const double *__restr
         return;
     }
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    constexpr int kRowsPerWave = PairShape<NT>::kRowsPerWave;
     double q[kRowsPerWave];
 #pragma unroll
     for (int i = 0; i < kRowsPerWave; i++) q[i] = (wave * kRowsPerWave + i == lane) ? 1.0 : 0.0;
-    const int rot = pair_jacobi(S, q, tl, inner_sweeps, ut, &sh_cnt, sh_c, sh_s, sh_pa, sh_pb);
+    const int rot = pair_jacobi<NT>(S, q, tl, inner_sweeps, ut, &sh_cnt, sh_c, sh_s, sh_pa, sh_pb);
     double *Qo = Qbuf + B.qoff + (int64_t)k * kP * kP;
 #pragma unroll
     for (int i = 0; i < kRowsPerWave; i++) Qo[(wave * kRowsPerWave + i) * kP + lane] = q[i];
@@ -700,10 +707,14 @@ void eig_tol(const EigBlock *blocks, int nblocks, const double *Mw, double *tol,
 
 void eig_pairs(const EigBlock *blocks, int nblocks, int maxpairs, int round, const double *Aw,
                double *Qbuf, int *active, const double *tol, int *counter, int inner_sweeps,
-               hipStream_t st)
+               int wide, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_pair_eig, dim3(maxpairs, nblocks), dim3(kTJ), 0, st, blocks, round, Aw, Qbuf,
-                       active, tol, counter, inner_sweeps);
+    if (wide)
+        hipLaunchKernelGGL(k_pair_eig<1024>, dim3(maxpairs, nblocks), dim3(1024), 0, st, blocks, round,
+                           Aw, Qbuf, active, tol, counter, inner_sweeps);
+    else
+        hipLaunchKernelGGL(k_pair_eig<256>, dim3(maxpairs, nblocks), dim3(256), 0, st, blocks, round,
+                           Aw, Qbuf, active, tol, counter, inner_sweeps);
 }
 
 void eig_apply(const EigBlock *blocks, int nblocks, int64_t maxtiles, int round, double *Aw,

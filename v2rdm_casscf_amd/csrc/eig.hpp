@@ -36,10 +36,11 @@ void eig_build_m(const EigBlock *blocks, int nblocks, int max_N, const double *a
 void eig_tol(const EigBlock *blocks, int nblocks, const double *Mw, double *tol, double *part,
              hipStream_t st);
 // one block-Jacobi round: the pair sub-problems (inner_sweeps caps the cyclic
-// sweeps of each 64 x 64 pair), then the rotation of the A and V tiles
+// sweeps of each 64 x 64 pair; wide = 1: 1024-thread workgroups, for problems
+// whose rounds are latency-bound), then the rotation of the A and V tiles
 void eig_pairs(const EigBlock *blocks, int nblocks, int maxpairs, int round, const double *Aw,
                double *Qbuf, int *active, const double *tol, int *counter, int inner_sweeps,
-               hipStream_t st);
+               int wide, hipStream_t st);
 void eig_apply(const EigBlock *blocks, int nblocks, int64_t maxtiles, int round, double *Aw,
                double *Vw, const double *Qbuf, const int *active, hipStream_t st);
 void eig_scale(const EigBlock *blocks, int nblocks, const double *Aw, const double *Vw, double *Tw,
