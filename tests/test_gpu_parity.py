@@ -117,6 +117,30 @@ def test_update_xz_matches_oracle(amopi, na, nb, mult, pos, sz):
     assert _rel(z.cpu().numpy(), zr2) < 1e-10
 
 
+def test_update_xz_large_blocks_match_oracle():
+    """Blocks padded to 512 or more take the latency-bound launch shape: two
+    block-Jacobi lanes on two streams and 1024-thread pair workgroups.  C1 with
+    17 orbitals: the G2 super block is 578 (padded 640), five blocks of 289
+    (320) and four of 136 (192) share the two lanes."""
+    amopi, na, nb = [17], 4, 4
+    s, L = _pair(amopi, na, nb, 1, "DQG", True)
+    assert max(L.block_dims) > 512
+    rng = np.random.default_rng(11)
+    y = rng.normal(size=L.ncon)
+    c = rng.normal(size=L.dimx)
+    x0 = rng.normal(size=L.dimx)
+    xr, zr = L.update_xz(0.8, y, c, x0)
+    x, z = _dev(x0.copy()), _dev(np.zeros(L.dimx))
+    s.update_xz(0.8, _dev(y), _dev(c), x, z)
+    assert _rel(x.cpu().numpy(), xr) < 1e-10
+    assert _rel(z.cpu().numpy(), zr) < 1e-10
+    y2 = y + 1e-3 * rng.normal(size=L.ncon)
+    xr2, zr2 = L.update_xz(0.8, y2, c, xr)
+    s.update_xz(0.8, _dev(y2), _dev(c), x, z)
+    assert _rel(x.cpu().numpy(), xr2) < 1e-10
+    assert _rel(z.cpu().numpy(), zr2) < 1e-10
+
+
 @pytest.mark.parametrize("amopi,na,nb,pos", [([4], 2, 2, "DQG"), ([2, 1, 1, 0], 2, 2, "DQG"),
                                              ([5], 3, 2, "DQG")])
 def test_bpsdp_energy_matches_oracle(amopi, na, nb, pos):

@@ -583,7 +583,7 @@ void v2g_solver::update_xz(double mu, const double *y, const double *c, double *
     int sweeps = 0;
     for (; sweeps < maxsweeps; sweeps++) {
         HIPCHK(hipMemsetAsync(counter.p, 0, sizeof(int), st));
-        if (nlanes > 1) {
+        if (nlanes > 2) {   // lanes 2, 4, ... start from the engine stream's state
             HIPCHK(hipEventRecord(ev_fork, st));
             for (int l = 2; l < nlanes; l += 2) HIPCHK(hipStreamWaitEvent(lane_st[l], ev_fork, 0));
         }

@@ -205,19 +205,6 @@ __device__ int pair_jacobi(double *S, double (&q)[PairShape<NT>::kRowsPerWave], 
     for (int sweep = 0; sweep < maxsweeps; sweep++) {
         if (threadIdx.x == 0) *sh_cnt = 0;
         __syncthreads();
-        // the circle positions of this thread's slots at step r, advanced by one
-        // (mod 63) per step instead of recomputed: slot 0 = (63, r), slot k > 0 =
-        // (r + k, r - k) mod 63
-        int ca[kUTper][2], cb[kUTper][2];
-#pragma unroll
-        for (int u = 0; u < kUTper; u++) {
-            const int sl[2] = {myP[u] < 0 ? 0 : myP[u], myR[u] < 0 ? 0 : myR[u]};
-#pragma unroll
-            for (int v = 0; v < 2; v++) {
-                ca[u][v] = sl[v] == 0 ? kP - 1 : sl[v];
-                cb[u][v] = sl[v] == 0 ? 0 : kP - 1 - sl[v];
-            }
-        }
         for (int r = 0; r < kP - 1; r++) {
             const int pl = circle_partner(r, lane);
             if (wave == 0) {
@@ -258,8 +245,9 @@ __device__ int pair_jacobi(double *S, double (&q)[PairShape<NT>::kRowsPerWave], 
                         cf[u][2] = sh_c[R];
                         cf[u][3] = sh_s[R];
                         act[u] = !(cf[u][1] == 0.0 && cf[u][3] == 0.0);
-                        const int p0 = min(ca[u][0], cb[u][0]), p1 = max(ca[u][0], cb[u][0]);
-                        const int r0 = min(ca[u][1], cb[u][1]), r1 = max(ca[u][1], cb[u][1]);
+                        int p0, p1, r0, r1;
+                        slot_pair(r, P, p0, p1);
+                        slot_pair(r, R, r0, r1);
                         auto ad = [](int i, int j) { return i < j ? i * kLd + j : j * kLd + i; };
                         addr[u][0] = ad(p0, r0);
                         addr[u][1] = ad(p0, r1);
@@ -298,14 +286,6 @@ __device__ int pair_jacobi(double *S, double (&q)[PairShape<NT>::kRowsPerWave], 
 #pragma unroll
                 for (int i = 0; i < kRowsPerWave; i++) q[i] = c * q[i] + sn * o[i];
             }
-            // advance the slot positions to step r + 1
-#pragma unroll
-            for (int u = 0; u < kUTper; u++)
-#pragma unroll
-                for (int v = 0; v < 2; v++) {
-                    if (ca[u][v] != kP - 1) ca[u][v] = ca[u][v] == kP - 2 ? 0 : ca[u][v] + 1;
-                    cb[u][v] = cb[u][v] == kP - 2 ? 0 : cb[u][v] + 1;
-                }
             __syncthreads();
         }
         const int cnt = *sh_cnt;
