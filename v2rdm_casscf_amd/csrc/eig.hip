@@ -143,7 +143,8 @@ __device__ int lds_jacobi(double *S, double *Q, int m, double tol, int maxsweeps
 //    partner lane of the same wave, so Q moves through lane shuffles and never
 //    touches LDS.  That leaves one 64 x 65 FP64 array of LDS per workgroup.
 // In step r of the circle method, lane l is paired with 63 <-> r and
-// l <-> (2r - l) mod 63 otherwise; kk(r, l) is the pair's slot.
+// l <-> (2r - l) mod 63 otherwise; circle_slot(r, l) is the pair's slot (the
+// k of circle_pair(kP, r, k)).
 // x mod 63 for 0 <= x < 189 without a division
 __device__ __forceinline__ int mod63(int x)
 {
@@ -165,16 +166,6 @@ __device__ __forceinline__ int circle_slot(int r, int l)
     return d <= kP / 2 - 1 ? d : mod63(r - l + (kP - 1));
 }
 
-// the (smaller, larger) index pair of slot k in step r, as circle_pair(kP, r, k)
-__device__ __forceinline__ void slot_pair(int r, int k, int &p, int &q)
-{
-    int a, b;
-    if (k == 0) { a = kP - 1; b = r; }
-    else { a = mod63(r + k); b = mod63(r - k + (kP - 1)); }
-    p = a < b ? a : b;
-    q = a < b ? b : a;
-}
-
 constexpr int kUT = (kP / 2) * (kP / 2 + 1) / 2;   // 528 pair blocks with P <= R
 // Workgroup size NT of the pair problems.  Large problems have a few hundred
 // pair problems per round, so a step's latency sets the time: NT = 1024 (16
@@ -186,10 +177,13 @@ template <int NT> struct PairShape {
     static constexpr int kUTper = (kUT + NT - 1) / NT;  // S blocks per thread
 };
 
+// Per step, wave 0 publishes each slot's rotation (sh_c, sh_s) and index pair
+// (sh_pa < sh_pb) for the S update, and each lane's own signed rotation
+// (sh_cl, sh_sl) for the Q update, so no other wave recomputes circle positions.
 template <int NT>
 __device__ int pair_jacobi(double *S, double (&q)[PairShape<NT>::kRowsPerWave], double tol,
                            int maxsweeps, const unsigned short *ut, int *sh_cnt, double *sh_c,
-                           double *sh_s, int *sh_pa, int *sh_pb)
+                           double *sh_s, int *sh_pa, int *sh_pb, double *sh_cl, double *sh_sl)
 {
     constexpr int kRowsPerWave = PairShape<NT>::kRowsPerWave, kUTper = PairShape<NT>::kUTper;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -223,6 +217,11 @@ __device__ int pair_jacobi(double *S, double (&q)[PairShape<NT>::kRowsPerWave], 
                     sh_s[k] = sn;
                     sh_pa[k] = p;
                     sh_pb[k] = qq;
+                    // column p becomes c q_p - s q_qq, column qq c q_qq + s q_p
+                    sh_cl[p] = c;
+                    sh_sl[p] = -sn;
+                    sh_cl[qq] = c;
+                    sh_sl[qq] = sn;
                 }
                 const unsigned long long b = __ballot(rotated);
                 if (lane == 0 && b) *sh_cnt += __popcll(b);
@@ -245,9 +244,7 @@ __device__ int pair_jacobi(double *S, double (&q)[PairShape<NT>::kRowsPerWave], 
                         cf[u][2] = sh_c[R];
                         cf[u][3] = sh_s[R];
                         act[u] = !(cf[u][1] == 0.0 && cf[u][3] == 0.0);
-                        int p0, p1, r0, r1;
-                        slot_pair(r, P, p0, p1);
-                        slot_pair(r, R, r0, r1);
+                        const int p0 = sh_pa[P], p1 = sh_pb[P], r0 = sh_pa[R], r1 = sh_pb[R];
                         auto ad = [](int i, int j) { return i < j ? i * kLd + j : j * kLd + i; };
                         addr[u][0] = ad(p0, r0);
                         addr[u][1] = ad(p0, r1);
@@ -276,8 +273,7 @@ __device__ int pair_jacobi(double *S, double (&q)[PairShape<NT>::kRowsPerWave], 
             }
             // Q <- Q R in registers; every lane takes part in the shuffles
             {
-                const int k = circle_slot(r, lane);
-                const double c = sh_c[k], sn = lane < pl ? -sh_s[k] : sh_s[k];
+                const double c = sh_cl[lane], sn = sh_sl[lane];
                 // all shuffles first, then the arithmetic: one wait for the
                 // whole batch instead of one per row
                 double o[kRowsPerWave];
@@ -458,6 +454,7 @@ k_pair_eig(const EigBlock *__restrict__ blocks, int round, const double *__restr
     __shared__ double S[kP * kLd];
     __shared__ double sh_c[kP / 2], sh_s[kP / 2];
     __shared__ int sh_pa[kP / 2], sh_pb[kP / 2], sh_cnt;
+    __shared__ double sh_cl[kP], sh_sl[kP];
     __shared__ unsigned short ut[kUT];
     const EigBlock B = blocks[blockIdx.y];
     const int nbk = B.N / 32;
@@ -504,7 +501,8 @@ k_pair_eig(const EigBlock *__restrict__ blocks, int round, const double *__restr
     double q[kRowsPerWave];
 #pragma unroll
     for (int i = 0; i < kRowsPerWave; i++) q[i] = (wave * kRowsPerWave + i == lane) ? 1.0 : 0.0;
-    const int rot = pair_jacobi<NT>(S, q, tl, inner_sweeps, ut, &sh_cnt, sh_c, sh_s, sh_pa, sh_pb);
+    const int rot = pair_jacobi<NT>(S, q, tl, inner_sweeps, ut, &sh_cnt, sh_c, sh_s, sh_pa, sh_pb,
+                                    sh_cl, sh_sl);
     double *Qo = Qbuf + B.qoff + (int64_t)k * kP * kP;
 #pragma unroll
     for (int i = 0; i < kRowsPerWave; i++) Qo[(wave * kRowsPerWave + i) * kP + lane] = q[i];
