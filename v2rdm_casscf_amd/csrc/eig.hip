@@ -552,8 +552,9 @@ __device__ __forceinline__ void store_tile(double *T, const v4d (&acc)[2][2])
 }
 
 // A_PR <- Q_P^T A_PR Q_R for the pair tiles with P <= R (A is symmetric: the
-// (R, P) tile is written as the transpose); V[:, R] <- V[:, R] Q_R
-__global__ void __launch_bounds__(kT)
+// (R, P) tile is written as the transpose); V[:, R] <- V[:, R] Q_R.  Four waves per SIMD: the LDS tile allows four
+// workgroups per CU, and the bound keeps the registers within that (116 VGPRs)
+__global__ void __launch_bounds__(kT, 4)
 k_apply(const EigBlock *__restrict__ blocks, int round, double *__restrict__ Aw,
         double *__restrict__ Vw, const double *__restrict__ Qbuf, const int *__restrict__ active)
 {

@@ -19,7 +19,8 @@ constexpr int kT = 256, kTile = 64, kK = 32;
 constexpr int kLoads = kK * kTile / kT;   // elements of each panel one thread moves
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-__global__ void __launch_bounds__(kT)
+// four waves per SIMD (the LDS panels allow four workgroups per CU; 126 VGPRs)
+__global__ void __launch_bounds__(kT, 4)
 k_gemm(const GemmDesc *__restrict__ descs)
 {
     __shared__ double As[kK][kTile + 1];
