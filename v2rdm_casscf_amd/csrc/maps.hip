@@ -200,7 +200,8 @@ k_row_norm2(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
     }
 }
 
-__global__ void __launch_bounds__(kBlock)
+// eight waves per SIMD (57 VGPRs) to hide the gathers' latency
+__global__ void __launch_bounds__(kBlock, 8)
 k_atu(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
       const int64_t *__restrict__ tile_start, int nseg, const double *__restrict__ y,
       double *__restrict__ ATy, const double *__restrict__ done, int64_t ntiles)
