@@ -193,8 +193,9 @@ def main():
                          "of 8 (dimx + ncon) bytes as the traffic reference, for counter runs")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: every box in this setup has a single MI355X; the "
-              "multi-GPU sharding plan is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py measures one MI355X (--gpus 1); --gpus {args.gpus} is not supported: "
+              "the engine runs one process on one GPU and has no multi-GPU path",
+              file=sys.stderr)
         sys.exit(2)
 
     import torch

@@ -854,7 +854,7 @@ int v2g_bpsdp(v2g_handle h, const v2g_params *prm, const double *c, const double
             h->mlp_outer_trained++;
             last_jacobi_iters = iiter;
         } else if (pc == 4) {
-            // the true residual is the judge: abandon the learned diagonal if it
+            // the true residual decides: abandon the learned diagonal if it
             // failed to converge, or if its first solve needed clearly more
             // iterations than the last Jacobi solve
             if (iiter >= prm->cg_maxiter || !std::isfinite(res)) fallback = true;
