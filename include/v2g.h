@@ -73,6 +73,19 @@ int v2g_update_xz(v2g_handle h, double mu, const double *y_dev, const double *c_
 /* forget the kept eigenvectors (next update_xz starts cold) */
 int v2g_reset_eig(v2g_handle h);
 
+/* ---- launch shape of the eigen split (no reference counterpart) ----
+ * lanes: 1 or 2 groups of blocks advancing their block-Jacobi rounds on their
+ * own streams; pair_threads: 256 or 1024 threads per 64 x 64 pair
+ * sub-problem.  0 keeps the current value.  Every shape gives the same split
+ * bit for bit; the default follows the largest padded block (two lanes and
+ * 1024 threads from 512 on). */
+int v2g_set_eig_shape(v2g_handle h, int lanes, int pair_threads);
+int v2g_get_eig_shape(v2g_handle h, int *lanes, int *pair_threads);
+/* on != 0: the next warm splits time every shape (two calls each, block-Jacobi
+ * rounds only, HIP events) and keep the fastest per sweep; the choice is read
+ * back with v2g_get_eig_shape.  on = 0 stops a tuning in progress. */
+int v2g_set_autotune(v2g_handle h, int on);
+
 /* ---- learned preconditioner (no reference counterpart) ----
  * A per-row MLP (14 inputs: one-hot constraint family x11, log diag(A A^T),
  * asinh(rhs/rms), asinh(y_prev/rms); tanh hidden layers 64, 64; one output g)
@@ -115,6 +128,14 @@ typedef struct v2g_params {
     int cg_extrapolate;          /* 1: start each CG from 2 y_k - y_(k-1)  */
                                  /* (linear extrapolation of the last two */
                                  /* solutions); 0: from y_k (reference)   */
+    /* resume (a solve continued by a second call, or from a checkpoint):
+     * the state of the last iteration done, as the previous call's stats
+     * report it; all zero for a fresh solve (mu starts at 1, as in
+     * v2rdm_solver.cc:1735) */
+    double mu0;                  /* > 0: mu to continue with               */
+    int oiter0;                  /* outer iterations already done          */
+    double ep0, ed0;             /* their last primal / dual errors (CG    */
+                                 /* tolerance of the first iteration)      */
 } v2g_params;
 
 typedef struct v2g_stats {
@@ -129,6 +150,9 @@ typedef struct v2g_stats {
     int train_outer;             /* outer iterations in the training phase */
     double mlp_loss;             /* last training loss                     */
     int fallback;                /* 1 if the learned PC was abandoned      */
+    int exact_splits;            /* iterations whose tracked split was     */
+                                 /* redone to convergence before the       */
+                                 /* stopping rule was applied              */
 } v2g_stats;
 
 /* v2RDMSolver::compute_energy outer loop (v2rdm_solver.cc:1778-1912, orbital

@@ -29,7 +29,9 @@ class Params(ctypes.Structure):
                 ("max_seconds", ctypes.c_double), ("train_iters", ctypes.c_int),
                 ("train_batch", ctypes.c_int), ("train_steps", ctypes.c_int),
                 ("train_lr", ctypes.c_double), ("eig_warm_sweeps", ctypes.c_int),
-                ("eig_inner_sweeps", ctypes.c_int), ("cg_extrapolate", ctypes.c_int)]
+                ("eig_inner_sweeps", ctypes.c_int), ("cg_extrapolate", ctypes.c_int),
+                ("mu0", ctypes.c_double), ("oiter0", ctypes.c_int), ("ep0", ctypes.c_double),
+                ("ed0", ctypes.c_double)]
 
 
 class Stats(ctypes.Structure):
@@ -40,7 +42,8 @@ class Stats(ctypes.Structure):
                 ("t_xz", ctypes.c_double), ("t_total", ctypes.c_double),
                 ("eig_sweeps", ctypes.c_int), ("precond_active", ctypes.c_int),
                 ("cg_iters_train", ctypes.c_longlong), ("train_outer", ctypes.c_int),
-                ("mlp_loss", ctypes.c_double), ("fallback", ctypes.c_int)]
+                ("mlp_loss", ctypes.c_double), ("fallback", ctypes.c_int),
+                ("exact_splits", ctypes.c_int)]
 
 
 ITER_CALLBACK = ctypes.CFUNCTYPE(ctypes.c_int, ctypes.c_void_p, ctypes.c_int,
@@ -64,6 +67,9 @@ SIGNATURES = {
     "v2g_set_precond_diag": (ctypes.c_int, [_VP, _DP]),
     "v2g_update_xz": (ctypes.c_int, [_VP, ctypes.c_double, _DP, _DP, _DP, _DP]),
     "v2g_reset_eig": (ctypes.c_int, [_VP]),
+    "v2g_set_eig_shape": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int]),
+    "v2g_get_eig_shape": (ctypes.c_int, [_VP, _IP, _IP]),
+    "v2g_set_autotune": (ctypes.c_int, [_VP, ctypes.c_int]),
     "v2g_bpsdp": (ctypes.c_int, [_VP, ctypes.POINTER(Params), _DP, _DP, _DP, _DP, _DP,
                                  ctypes.POINTER(Stats)]),
     "v2g_set_iter_callback": (ctypes.c_int, [_VP, ITER_CALLBACK, _VP]),

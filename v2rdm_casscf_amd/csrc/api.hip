@@ -83,6 +83,7 @@ struct v2g_solver {
     // loop scratch
     DevBuf<double> Ax, Bv, ATy, red;
     DevBuf<double> yhist;   // previous outer iteration's CG solution
+    DevBuf<double> xprev;   // x on entry to the last in-loop split
     int mlp_outer_trained = 0;  // outer iterations of MLP training done so far
     bool mlp_fallback = false, mlp_first_learned = true;
     int mlp_last_base_iters = 0;   // CG iterations of the last training solve
@@ -111,8 +112,23 @@ struct v2g_solver {
     int64_t lane_tiles[kMaxLanes] = {};
     hipStream_t lane_st[kMaxLanes] = {};
     hipEvent_t ev_fork = nullptr, ev_start[kMaxLanes] = {}, ev_join[kMaxLanes] = {};
+    std::vector<v2g::EigBlock> large_nat;   // the large blocks in dimensions_ order
+    // 1024-thread pair workgroups (latency-bound rounds) instead of 256
+    bool pair_wide = false;
     bool warm = false;
     int last_sweeps = 0;
+    // warm calls whose sweeps were capped and still rotated: the split is an
+    // approximation of the eigendecomposition (tracking), not a converged one
+    bool last_split_capped = false;
+    const double *rec_x = nullptr;          // x the reconstruction descriptors point at
+    // autotuner: the warm splits of the first calls try every launch shape
+    // (lanes, pair workgroup) kTuneReps times each and keep the fastest per sweep
+    static constexpr int kTuneReps = 2;
+    bool tuning = false;
+    int tune_k = 0;
+    std::vector<std::pair<int, int>> tune_cand;   // (lanes, 1024-thread pairs)
+    std::vector<float> tune_ms;                  // best ms per sweep per candidate
+    hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;
     v2g_iter_callback cb = nullptr;
     void *cb_user = nullptr;
     // learned preconditioner
@@ -139,6 +155,8 @@ struct v2g_solver {
     {
         if (cg_exec) (void)hipGraphExecDestroy(cg_exec);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_t0) (void)hipEventDestroy(ev_t0);
+        if (ev_t1) (void)hipEventDestroy(ev_t1);
         for (int l = 0; l < kMaxLanes; l++) {
             if (ev_start[l]) (void)hipEventDestroy(ev_start[l]);
             if (ev_join[l]) (void)hipEventDestroy(ev_join[l]);
@@ -146,6 +164,9 @@ struct v2g_solver {
         }
         if (own) (void)hipStreamDestroy(own);
     }
+    void configure_lanes(int want);
+    void set_shape(int lanes, int wide);
+    void start_tuning();
     void ensure_jacobi();
     void ensure_mlp();
     void mlp_init(unsigned long long seed);
@@ -153,8 +174,15 @@ struct v2g_solver {
     void mlp_infer(const double *rhs, const double *y);
     double mlp_train(const double *rhs, const double *rsample, int nb, int steps, double lr,
                      std::mt19937_64 &rng);
+    // the device vectors and scalars one CG solve works in
+    struct CgBufs {
+        double *r, *p, *Ap, *t, *partial, *scal, *rowpart;
+    };
+    CgBufs own_cg_bufs() { return {r.p, p.p, Ap.p, t.p, partial.p, scal.p, rowpart.p}; }
     int cg(const double *rhs, double *y, double tol, int maxiter, int precond, int flexible,
            double *res, double *r0_out = nullptr);
+    int cg_with(const CgBufs &B, bool graphs, const double *rhs, double *y, double tol, int maxiter,
+                const double *minv, int flexible, double *res, double *r0_out);
     void update_xz(double mu, const double *y, const double *c, double *x, double *z,
                    int warm_sweep_cap = 0, int inner_sweep_cap = 0);
     double dot(const double *a, const double *b, int64_t n);
@@ -290,44 +318,12 @@ v2g_solver::v2g_solver(const v2g_layout_desc &d)
         max_N = std::max(max_N, e.N);
         large.push_back(e);
     }
-    // lanes by longest-processing-time on N^3 (largest blocks first), for
-    // problems whose rounds are long enough (N >= kLaneMinN) to pay for the
-    // extra launches
-    if (large.size() >= 2 && max_N >= kLaneMinN) {
-        nlanes = std::min<int>(kMaxLanes, (int)large.size());
-        std::vector<v2g::EigBlock> sorted = large, g[kMaxLanes];
-        std::stable_sort(sorted.begin(), sorted.end(),
-                         [](const v2g::EigBlock &x, const v2g::EigBlock &y) { return x.N > y.N; });
-        double load[kMaxLanes] = {};
-        for (const auto &e : sorted) {
-            int l = 0;
-            for (int k = 1; k < nlanes; k++)
-                if (load[k] < load[l]) l = k;
-            g[l].push_back(e);
-            load[l] += (double)e.N * e.N * e.N;
-        }
-        large.clear();
-        for (int l = 0; l < nlanes; l++) {
-            lane_off[l] = (int)large.size();
-            lane_n[l] = (int)g[l].size();
-            large.insert(large.end(), g[l].begin(), g[l].end());
-            if (l > 0) HIPCHK(hipStreamCreateWithFlags(&lane_st[l], hipStreamNonBlocking));
-            HIPCHK(hipEventCreateWithFlags(&ev_start[l], hipEventDisableTiming));
-            HIPCHK(hipEventCreateWithFlags(&ev_join[l], hipEventDisableTiming));
-        }
-        HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
-    } else {
-        lane_n[0] = (int)large.size();
-    }
-    for (int l = 0; l < nlanes; l++)
-        for (int i = lane_off[l]; i < lane_off[l] + lane_n[l]; i++) {
-            const int np = large[i].N / 64;
-            lane_pairs[l] = std::max(lane_pairs[l], np);
-            lane_nb[l] = std::max(lane_nb[l], large[i].N / 32);
-            lane_tiles[l] = std::max(lane_tiles[l], (int64_t)np * (np + 1) / 2 + (int64_t)np * np);
-        }
+    large_nat = large;
+    // problem size decides the default launch shape: from a padded block of
+    // kLaneMinN on, the rounds are latency-bound (two lanes, 1024-thread pairs)
+    pair_wide = max_N >= kLaneMinN;
+    configure_lanes(max_N >= kLaneMinN ? kMaxLanes : 1);
     d_small.upload(small);
-    d_large.upload(large);
     if (!large.empty()) {
         Mw.alloc(woff);
         Aw.alloc(woff);
@@ -338,6 +334,77 @@ v2g_solver::v2g_solver(const v2g_layout_desc &d)
         tol.alloc(large.size());
         frop.alloc(64 * large.size());
         counter.alloc(1);
+    }
+}
+
+// Split the large blocks into `want` lanes of equal work (sum of N^3,
+// longest-processing-time order, largest blocks first).  Blocks are
+// independent, so every lane count gives the same split bit for bit; the
+// kept eigenvectors live at fixed workspace offsets and stay valid.
+void v2g_solver::configure_lanes(int want)
+{
+    if (want < 1 || want > kMaxLanes) throw std::invalid_argument("lanes must be 1 or 2");
+    HIPCHK(hipStreamSynchronize(st));
+    for (int l = 1; l < kMaxLanes; l++)
+        if (lane_st[l]) HIPCHK(hipStreamSynchronize(lane_st[l]));
+    nlanes = std::min<int>(want, std::max<int>(1, (int)large_nat.size()));
+    std::vector<v2g::EigBlock> sorted = large_nat, g[kMaxLanes];
+    std::stable_sort(sorted.begin(), sorted.end(),
+                     [](const v2g::EigBlock &x, const v2g::EigBlock &y) { return x.N > y.N; });
+    double load[kMaxLanes] = {};
+    for (const auto &e : sorted) {
+        int l = 0;
+        for (int k = 1; k < nlanes; k++)
+            if (load[k] < load[l]) l = k;
+        g[l].push_back(e);
+        load[l] += (double)e.N * e.N * e.N;
+    }
+    large.clear();
+    for (int l = 0; l < kMaxLanes; l++) {
+        lane_off[l] = (int)large.size();
+        lane_n[l] = l < nlanes ? (int)g[l].size() : 0;
+        lane_pairs[l] = lane_nb[l] = 0;
+        lane_tiles[l] = 0;
+        if (l < nlanes) large.insert(large.end(), g[l].begin(), g[l].end());
+        if (l < nlanes && nlanes > 1) {
+            if (l > 0 && !lane_st[l]) HIPCHK(hipStreamCreateWithFlags(&lane_st[l], hipStreamNonBlocking));
+            if (!ev_start[l]) HIPCHK(hipEventCreateWithFlags(&ev_start[l], hipEventDisableTiming));
+            if (!ev_join[l]) HIPCHK(hipEventCreateWithFlags(&ev_join[l], hipEventDisableTiming));
+        }
+    }
+    if (nlanes > 1 && !ev_fork) HIPCHK(hipEventCreateWithFlags(&ev_fork, hipEventDisableTiming));
+    for (int l = 0; l < nlanes; l++)
+        for (int i = lane_off[l]; i < lane_off[l] + lane_n[l]; i++) {
+            const int np = large[i].N / 64;
+            lane_pairs[l] = std::max(lane_pairs[l], np);
+            lane_nb[l] = std::max(lane_nb[l], large[i].N / 32);
+            lane_tiles[l] = std::max(lane_tiles[l], (int64_t)np * (np + 1) / 2 + (int64_t)np * np);
+        }
+    d_large.upload(large.empty() ? std::vector<v2g::EigBlock>{} : large);
+    // descriptors of the warm-start products follow the block order
+    d_gemm_warm1.alloc(0);
+    d_gemm_warm2.alloc(0);
+    rec_x = nullptr;
+}
+
+void v2g_solver::set_shape(int lanes, int wide)
+{
+    if (lanes > 0 && lanes != nlanes) configure_lanes(lanes);
+    if (wide >= 0) pair_wide = wide != 0;
+}
+
+void v2g_solver::start_tuning()
+{
+    tune_cand.clear();
+    const int maxl = std::min<int>(kMaxLanes, (int)large_nat.size());
+    for (int l = 1; l <= maxl; l++)
+        for (int w = 0; w < 2; w++) tune_cand.push_back({l, w});
+    tune_ms.assign(tune_cand.size(), 0.0f);
+    tune_k = 0;
+    tuning = tune_cand.size() > 1;
+    if (tuning && !ev_t0) {
+        HIPCHK(hipEventCreate(&ev_t0));
+        HIPCHK(hipEventCreate(&ev_t1));
     }
 }
 
@@ -478,7 +545,6 @@ double v2g_solver::dot(const double *a, const double *b, int64_t n)
 int v2g_solver::cg(const double *rhs, double *y, double tolv, int maxiter, int precond,
                    int flexible, double *res, double *r0_out)
 {
-    const int64_t n = L->ncon;
     const double *minv = nullptr;
     if (precond == 1) { ensure_jacobi(); minv = jacobi.p; }
     else if (precond == 2) {
@@ -487,27 +553,34 @@ int v2g_solver::cg(const double *rhs, double *y, double tolv, int maxiter, int p
     } else if (precond == 4) {
         minv = lminv.p;  // the learned diagonal from mlp_infer
     }
+    return cg_with(own_cg_bufs(), use_graphs, rhs, y, tolv, maxiter, minv, flexible, res, r0_out);
+}
+
+int v2g_solver::cg_with(const CgBufs &B, bool graphs, const double *rhs, double *y, double tolv,
+                        int maxiter, const double *minv, int flexible, double *res, double *r0_out)
+{
+    const int64_t n = L->ncon;
     // Ap = A A^T y ; r = b - Ap ; p = M^-1 r   (cg_solver.cc:141-150)
-    v2g::launch_atu(plan, y, t.p, st);
-    v2g::launch_au(plan, t.p, Ap.p, st);
-    v2g::cg_init(n, rhs, Ap.p, r.p, p.p, minv, nullptr, partial.p, scal.p, tolv, maxiter, st);
+    v2g::launch_atu(plan, y, B.t, st);
+    v2g::launch_au(plan, B.t, B.Ap, st);
+    v2g::cg_init(n, rhs, B.Ap, B.r, B.p, minv, nullptr, B.partial, B.scal, tolv, maxiter, st);
     if (r0_out)
-        HIPCHK(hipMemcpyAsync(r0_out, r.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        HIPCHK(hipMemcpyAsync(r0_out, B.r, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     // one chunk of kCgChunk (even) iterations; every kernel argument is fixed,
     // the iteration number and the done flags live in scal
     auto enqueue_chunk = [&]() {
         for (int c = 0; c < kCgChunk; c++) {
             const int par = c & 1;
-            const double *done = scal.p + v2g::CG_DONE0 + par;
-            v2g::launch_atu(plan, p.p, t.p, st, done);
-            v2g::launch_au(plan, t.p, Ap.p, st, done, p.p, rowpart.p);   // Ap and p.Ap
-            v2g::cg_dot_from_rows(plan.row_tiles, rowpart.p, partial.p, scal.p, par, st);
-            v2g::cg_step(n, r.p, Ap.p, minv, partial.p, scal.p, par, st);
-            v2g::cg_finish_iter(n, y, r.p, p.p, minv, partial.p, scal.p, par, flexible, st);
+            const double *done = B.scal + v2g::CG_DONE0 + par;
+            v2g::launch_atu(plan, B.p, B.t, st, done);
+            v2g::launch_au(plan, B.t, B.Ap, st, done, B.p, B.rowpart);   // Ap and p.Ap
+            v2g::cg_dot_from_rows(plan.row_tiles, B.rowpart, B.partial, B.scal, par, st);
+            v2g::cg_step(n, B.r, B.Ap, minv, B.partial, B.scal, par, st);
+            v2g::cg_finish_iter(n, y, B.r, B.p, minv, B.partial, B.scal, par, flexible, st);
         }
     };
     hipGraphExec_t exec = nullptr;
-    if (use_graphs && n <= kGraphMaxRows) {
+    if (graphs && n <= kGraphMaxRows) {
         if (!(cg_exec && cg_exec_y == y && cg_exec_minv == minv && cg_exec_flex == flexible)) {
             if (cg_exec) { (void)hipGraphExecDestroy(cg_exec); cg_exec = nullptr; }
             // relaxed mode: the other host threads of a bond scan keep allocating,
@@ -538,7 +611,7 @@ int v2g_solver::cg(const double *rhs, double *y, double tolv, int maxiter, int p
         if (exec) HIPCHK(hipGraphLaunch(exec, st));
         else enqueue_chunk();
         it += kCgChunk;
-        HIPCHK(hipMemcpyAsync(hs, scal.p, sizeof(hs), hipMemcpyDeviceToHost, st));
+        HIPCHK(hipMemcpyAsync(hs, B.scal, sizeof(hs), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         if (hs[v2g::CG_DONE0 + (it & 1)] != 0.0) break;
         if (it > maxiter + 2 * kCgChunk) break;  // defensive: cannot happen
@@ -558,13 +631,20 @@ void v2g_solver::update_xz(double mu, const double *y, const double *c, double *
     v2g::eig_small(d_small.p, (int)small.size(), ATy.p, c, x, zv, mu, st);
     last_sweeps = 0;
     if (large.empty()) return;
-    const int nl = (int)large.size();
     const int cold = warm ? 0 : 1;
+    // autotuning: this warm call runs candidate tune_k / kTuneReps
+    const bool timed = tuning && !cold;
+    if (timed) {
+        const auto &cd = tune_cand[tune_k / kTuneReps];
+        set_shape(cd.first, cd.second);
+    }
+    const int nl = (int)large.size();
     v2g::eig_build_m(d_large.p, nl, max_N, ATy.p, c, x, mu, Mw.p, Aw.p, Vw.p, cold, st);
     v2g::eig_tol(d_large.p, nl, Mw.p, tol.p, frop.p, st);
     if (!cold) {
         // A = V^T (M V)
         if (!d_gemm_warm1.p) {
+            HIPCHK(hipStreamSynchronize(st));
             std::vector<v2g::GemmDesc> g1, g2;
             for (const auto &b : large) {
                 g1.push_back({Mw.p + b.woff, Vw.p + b.woff, Tw.p + b.woff, b.N, b.N, b.N, b.N, b.N,
@@ -580,8 +660,10 @@ void v2g_solver::update_xz(double mu, const double *y, const double *c, double *
     }
     const int maxsweeps = (!cold && warm_sweep_cap > 0) ? warm_sweep_cap : 40;
     const int inner = (!cold && inner_sweep_cap > 0) ? inner_sweep_cap : 12;
-    int sweeps = 0;
+    int sweeps = 0, swept = 0;
+    if (timed) HIPCHK(hipEventRecord(ev_t0, st));
     for (; sweeps < maxsweeps; sweeps++) {
+        swept++;
         HIPCHK(hipMemsetAsync(counter.p, 0, sizeof(int), st));
         if (nlanes > 2) {   // lanes 2, 4, ... start from the engine stream's state
             HIPCHK(hipEventRecord(ev_fork, st));
@@ -593,7 +675,7 @@ void v2g_solver::update_xz(double mu, const double *y, const double *c, double *
                 const v2g::EigBlock *bl = d_large.p + lane_off[l];
                 hipStream_t s = l ? lane_st[l] : st;
                 v2g::eig_pairs(bl, lane_n[l], lane_pairs[l], rnd, Aw.p, Qbuf.p, active.p,
-                               tol.p + lane_off[l], counter.p, inner, max_N >= kLaneMinN, s);
+                               tol.p + lane_off[l], counter.p, inner, pair_wide ? 1 : 0, s);
                 if (rnd == 0 && l % 2 == 0 && l + 1 < nlanes) {
                     // lane l + 1 starts once lane l's first pair problems are
                     // done, so in steady state each of the two runs its tile
@@ -607,22 +689,43 @@ void v2g_solver::update_xz(double mu, const double *y, const double *c, double *
             HIPCHK(hipEventRecord(ev_join[l], lane_st[l]));
             HIPCHK(hipStreamWaitEvent(st, ev_join[l], 0));
         }
+        if (timed) HIPCHK(hipEventRecord(ev_t1, st));
         int cnt = 0;
         HIPCHK(hipMemcpyAsync(&cnt, counter.p, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHK(hipStreamSynchronize(st));
         if (cnt == 0) break;
     }
+    if (timed) {
+        float ms = 0.0f;
+        HIPCHK(hipEventElapsedTime(&ms, ev_t0, ev_t1));
+        ms /= (float)std::max(1, swept);
+        float &best = tune_ms[tune_k / kTuneReps];
+        if (tune_k % kTuneReps == 0 || ms < best) best = ms;
+        if (++tune_k == (int)tune_cand.size() * kTuneReps) {
+            int bi = 0;
+            for (int i = 1; i < (int)tune_cand.size(); i++)
+                if (tune_ms[i] < tune_ms[bi]) bi = i;
+            tuning = false;
+            set_shape(tune_cand[bi].first, tune_cand[bi].second);
+        }
+    }
     last_sweeps = sweeps;
+    last_split_capped = !cold && sweeps > 0 && (sweeps >= maxsweeps || inner < 12);
     // x = (V s)(V s)^T with s = sqrt(max(lambda,0)/mu); z = mu x - M
     v2g::eig_scale(d_large.p, nl, Aw.p, Vw.p, Tw.p, mu, st);
-    std::vector<v2g::GemmDesc> g3;
+    // the reconstruction descriptors point into x: built once per x buffer
+    // (an upload frees and allocates, which would drain the device every call)
     int maxn = 0;
-    for (const auto &b : large) {
-        g3.push_back({Tw.p + b.woff, Tw.p + b.woff, x + b.xoff, b.n, b.n, b.N, b.N, b.N, b.n, 0, 1,
-                      1});
-        maxn = std::max(maxn, b.n);
+    for (const auto &b : large) maxn = std::max(maxn, b.n);
+    if (rec_x != x || !d_gemm_rec.p) {
+        std::vector<v2g::GemmDesc> g3;
+        for (const auto &b : large)
+            g3.push_back({Tw.p + b.woff, Tw.p + b.woff, x + b.xoff, b.n, b.n, b.N, b.N, b.N, b.n, 0,
+                          1, 1});
+        HIPCHK(hipStreamSynchronize(st));
+        d_gemm_rec.upload(g3);
+        rec_x = x;
     }
-    d_gemm_rec.upload(g3);
     v2g::gemm_batched(d_gemm_rec.p, nl, maxn, maxn, st);
     v2g::eig_z(d_large.p, nl, x, Mw.p, zv, mu, st);
     warm = true;
@@ -766,6 +869,36 @@ int v2g_reset_eig(v2g_handle h)
     API_END
 }
 
+int v2g_set_eig_shape(v2g_handle h, int lanes, int pair_threads)
+{
+    API_BEGIN
+    if (!h) throw std::invalid_argument("null handle");
+    if (lanes != 0 && lanes != 1 && lanes != 2) throw std::invalid_argument("lanes must be 0, 1 or 2");
+    if (pair_threads != 0 && pair_threads != 256 && pair_threads != 1024)
+        throw std::invalid_argument("pair_threads must be 0, 256 or 1024");
+    h->tuning = false;
+    h->set_shape(lanes, pair_threads == 0 ? -1 : (pair_threads == 1024 ? 1 : 0));
+    API_END
+}
+
+int v2g_get_eig_shape(v2g_handle h, int *lanes, int *pair_threads)
+{
+    API_BEGIN
+    if (!h) throw std::invalid_argument("null handle");
+    if (lanes) *lanes = h->nlanes;
+    if (pair_threads) *pair_threads = h->pair_wide ? 1024 : 256;
+    API_END
+}
+
+int v2g_set_autotune(v2g_handle h, int on)
+{
+    API_BEGIN
+    if (!h) throw std::invalid_argument("null handle");
+    if (on) h->start_tuning();
+    else h->tuning = false;
+    API_END
+}
+
 int v2g_set_iter_callback(v2g_handle h, v2g_iter_callback cb, void *user)
 {
     API_BEGIN
@@ -783,8 +916,19 @@ int v2g_bpsdp(v2g_handle h, const v2g_params *prm, const double *c, const double
     const int64_t ncon = h->L->ncon, dimx = h->L->dimx;
     hipStream_t st = h->st;
     const double tau = prm->tau;
-    double mu = 1.0, ep = 0.0, ed = 0.0, egap = 0.0, ecur = 0.0, edual = 0.0;
-    int oiter = 0;
+    // a resumed solve (oiter0 > 0) continues with the mu, errors and iteration
+    // count of the call it follows
+    double mu = prm->mu0 > 0.0 ? prm->mu0 : 1.0, ep = prm->ep0, ed = prm->ed0, egap = 0.0,
+           ecur = 0.0, edual = 0.0;
+    if (prm->oiter0 < 0) throw std::invalid_argument("oiter0 must be >= 0");
+    int oiter = prm->oiter0;
+    const int oiter_first = oiter;
+    int exact_splits = 0;
+    // the in-loop split tracks the eigenvectors with capped sweeps; before the
+    // stopping rule may accept an iteration, that iteration's split is redone
+    // to convergence from the same x, so the answer is the reference's fixed point
+    const bool capped_loop = prm->eig_warm_sweeps > 0 || prm->eig_inner_sweeps > 0;
+    if (capped_loop && !h->xprev.p) h->xprev.alloc(dimx);
     long long cgtot = 0;
     double tcg = 0.0, txz = 0.0;
     const double t0 = now();
@@ -808,7 +952,7 @@ int v2g_bpsdp(v2g_handle h, const v2g_params *prm, const double *c, const double
         v2g::launch_au(h->plan, z, h->Bv.p, st);
         v2g::vec_axpby(ncon, 1.0, h->Bv.p, 1.0, h->Ax.p, h->Bv.p, st);
         double cgtol;  // v2rdm_solver.cc:1797-1805
-        if (oiter == 0) cgtol = 0.01;
+        if (oiter == 0 || (oiter == oiter_first && !(ep > 0.0 && ed > 0.0))) cgtol = 0.01;
         else cgtol = (ep > ed) ? 0.01 * ed : 0.01 * ep;
         if (cgtol < prm->cg_convergence) cgtol = prm->cg_convergence;
         double res = 0.0;
@@ -831,7 +975,7 @@ int v2g_bpsdp(v2g_handle h, const v2g_params *prm, const double *c, const double
         // without changing the system, the tolerance or the outer iteration
         if (prm->cg_extrapolate) {
             if (!h->yhist.p) h->yhist.alloc(ncon);
-            if (oiter == 0)
+            if (oiter == oiter_first)
                 HIPCHK(hipMemcpyAsync(h->yhist.p, y, sizeof(double) * ncon, hipMemcpyDeviceToDevice,
                                       st));
             else
@@ -864,20 +1008,40 @@ int v2g_bpsdp(v2g_handle h, const v2g_params *prm, const double *c, const double
         s.precond_active = pc == 4 ? 3 : pc;
         const double t2 = now();
         tcg += t2 - t1;
+        if (capped_loop)
+            HIPCHK(hipMemcpyAsync(h->xprev.p, x, sizeof(double) * dimx, hipMemcpyDeviceToDevice, st));
         h->update_xz(mu, y, c, x, z, prm->eig_warm_sweeps, prm->eig_inner_sweeps);
         HIPCHK(hipStreamSynchronize(st));
         txz += now() - t2;
         // ed = ||A^T y - c + z||, ep = ||A x - b||  (v2rdm_solver.cc:1829-1837)
-        v2g::launch_atu(h->plan, y, h->ATy.p, st);
-        v2g::vec_axpby(dimx, 1.0, h->ATy.p, 1.0, z, h->ATy.p, st);
-        v2g::vec_axpby(dimx, 1.0, h->ATy.p, -1.0, c, h->ATy.p, st);
-        ed = std::sqrt(h->dot(h->ATy.p, h->ATy.p, dimx));
-        v2g::launch_au(h->plan, x, h->Ax.p, st);
-        v2g::vec_axpby(ncon, 1.0, h->Ax.p, -1.0, b, h->Ax.p, st);
-        ep = std::sqrt(h->dot(h->Ax.p, h->Ax.p, ncon));
+        auto errors = [&]() {
+            v2g::launch_atu(h->plan, y, h->ATy.p, st);
+            v2g::vec_axpby(dimx, 1.0, h->ATy.p, 1.0, z, h->ATy.p, st);
+            v2g::vec_axpby(dimx, 1.0, h->ATy.p, -1.0, c, h->ATy.p, st);
+            ed = std::sqrt(h->dot(h->ATy.p, h->ATy.p, dimx));
+            v2g::launch_au(h->plan, x, h->Ax.p, st);
+            v2g::vec_axpby(ncon, 1.0, h->Ax.p, -1.0, b, h->Ax.p, st);
+            ep = std::sqrt(h->dot(h->Ax.p, h->Ax.p, ncon));
+            ecur = h->dot(c, x, dimx);
+            edual = h->dot(b, y, ncon);
+        };
+        errors();
+        auto passes = [&]() {
+            return !(ep > prm->r_convergence || ed > prm->r_convergence ||
+                     std::fabs(ecur - edual) > prm->e_convergence);
+        };
+        if (passes() && h->last_split_capped) {
+            // the tracked split rotated in its last sweep: redo this
+            // iteration's split to convergence (same y, same x on entry)
+            const double t3 = now();
+            HIPCHK(hipMemcpyAsync(x, h->xprev.p, sizeof(double) * dimx, hipMemcpyDeviceToDevice, st));
+            h->update_xz(mu, y, c, x, z, 0, 0);
+            HIPCHK(hipStreamSynchronize(st));
+            txz += now() - t3;
+            exact_splits++;
+            errors();
+        }
         if (oiter % prm->mu_update_frequency == 0 && oiter > 0) mu = mu * ep / ed;
-        ecur = h->dot(c, x, dimx);
-        edual = h->dot(b, y, ncon);
         if (prm->verbose)
             std::fprintf(stderr, "      %5i %5i %14.8f %14.8f %11.4e %7.3f %10.5e %10.5e  sweeps %d\n",
                          oiter, iiter, ecur, edual, std::fabs(ecur - edual), mu, ep, ed,
@@ -899,8 +1063,9 @@ int v2g_bpsdp(v2g_handle h, const v2g_params *prm, const double *c, const double
         s.train_outer = n_train;
         s.mlp_loss = last_loss;
         s.fallback = fallback ? 1 : 0;
+        s.exact_splits = exact_splits;
         if (h->cb && h->cb(h->cb_user, oiter, &s)) stop = true;
-        if (oiter == prm->maxiter) break;
+        if (oiter >= prm->maxiter) break;
         if (prm->max_seconds > 0 && now() - t0 > prm->max_seconds) break;
     } while (!stop && (ep > prm->r_convergence || ed > prm->r_convergence || egap > prm->e_convergence));
     s.converged = !(ep > prm->r_convergence || ed > prm->r_convergence || egap > prm->e_convergence);

@@ -99,20 +99,6 @@ __global__ void k_init_fin(double *__restrict__ partial, double *__restrict__ sc
     }
 }
 
-__global__ void __launch_bounds__(kThreads)
-k_dot(int64_t n, const double *__restrict__ a, const double *__restrict__ b,
-      double *__restrict__ partial, const double *__restrict__ scal, int parity)
-{
-    __shared__ double sh[kThreads / 64];
-    if (scal[S_DONE0 + parity] != 0.0) return;
-    double s = 0.0;
-    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * kThreads)
-        s += a[i] * b[i];
-    const double t = block_sum(s, sh);
-    if (threadIdx.x == 0) partial[P_PAP * kRedBlocks + blockIdx.x] = t;
-}
-
 // alpha = rz / pap; r -= alpha Ap; z = M^-1 r; partial r.z, r.r, r_old.z.
 // y += alpha p is left to k_finish, which reads p anyway (one vector pass less)
 __global__ void __launch_bounds__(kThreads)
@@ -270,13 +256,6 @@ void cg_init(int64_t n, const double *b, const double *Ap, double *r, double *p,
     hipLaunchKernelGGL(k_init, dim3(kRedBlocks), dim3(kThreads), 0, st, n, b, Ap, r, p, minv, z,
                        partial);
     hipLaunchKernelGGL(k_init_fin, dim3(1), dim3(kThreads), 0, st, partial, scal, tol, maxiter);
-}
-
-void cg_dot(int64_t n, const double *a, const double *b, double *partial, double *scal, int parity,
-            hipStream_t st)
-{
-    hipLaunchKernelGGL(k_dot, dim3(kRedBlocks), dim3(kThreads), 0, st, n, a, b, partial, scal,
-                       parity);
 }
 
 void cg_dot_from_rows(int64_t ntiles, const double *rowpart, double *partial, const double *scal,

@@ -49,8 +49,6 @@ constexpr int kPartialArrays = 4;   // partial buffer holds 4 * kRedBlocks doubl
 
 void cg_init(int64_t n, const double *b, const double *Ap, double *r, double *p, const double *minv,
              double *z, double *partial, double *scal, double tol, int maxiter, hipStream_t st);
-void cg_dot(int64_t n, const double *a, const double *b, double *partial, double *scal, int parity,
-            hipStream_t st);
 // p.Ap from the per-tile partials of the fused A.x launch
 void cg_dot_from_rows(int64_t ntiles, const double *rowpart, double *partial, const double *scal,
                       int parity, hipStream_t st);

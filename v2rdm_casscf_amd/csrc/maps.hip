@@ -5,16 +5,19 @@
 // the rows whose sums run over all orbitals (traces, S^2, D2->D1
 // contractions, maximal-spin rows) get one 64-lane wavefront each.
 //
-// A^T.y (bpsdp_ATu, v2rdm_solver.cc:3114-3158): also a gather, one thread per
-// primal entry, so the result is deterministic and needs no atomics:
+// A^T.y (bpsdp_ATu, v2rdm_solver.cc:3114-3158): for the D / Q / G / spin
+// rows also a gather, one thread per primal entry, deterministic and free of
+// atomics:
 //   * "self" blocks (D200, Q1, Q2, G2): each entry is touched by exactly one
 //     row with coefficient +-1, laid out identically -> a coalesced copy;
 //   * D2ab / D2aa / D2bb entries invert each family's index rule in closed
 //     form (the particle-hole G2 maps are permutations per irrep);
 //   * D1 entries sum O(amo) rows -> one wavefront per entry.
-// With T2 on, a second launch walks the T2 rows: the T2 block itself is a
-// self block, and the rare delta-selected D2 / D1 terms of a T2 row are added
-// with hardware FP64 atomics.
+// With T1 or T2 on, a second launch walks the three-body rows (T1 rows in a
+// triplet pass, T2 rows in a second pass): the T1 / T2 block itself is a self
+// block, and the terms those rows put into Q2 / D2 / G2 / D1 entries are added
+// with hardware FP64 atomics, so A^T.y with T1 or T2 is not bitwise
+// reproducible from run to run (it is to rounding).
 //
 // Row formulas follow d2.cc:296-547, spin.cc:274-493, q2.cc:801-950,
 // g2.cc:1061-1260 and t2.cc:1251-1861.

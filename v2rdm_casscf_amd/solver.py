@@ -86,6 +86,7 @@ class V2RDMSolver:
             self.options.update({k.upper(): v for k, v in options.items()})
         self.x = self.z = self.y = self.c = self.b = None
         self.mu = 1.0
+        self.last_stats = None
         self._cb = None
 
     def __del__(self):
@@ -206,6 +207,20 @@ class V2RDMSolver:
     def reset_eig(self):
         check(self.lib.v2g_reset_eig(self.h))
 
+    # ---- launch shape of the eigen split (no reference counterpart) ------
+    def set_eig_shape(self, lanes=0, pair_threads=0):
+        """lanes 1/2, pair_threads 256/1024 (0 keeps the current value)."""
+        check(self.lib.v2g_set_eig_shape(self.h, int(lanes), int(pair_threads)))
+
+    def eig_shape(self):
+        lanes, pt = ctypes.c_int(), ctypes.c_int()
+        check(self.lib.v2g_get_eig_shape(self.h, ctypes.byref(lanes), ctypes.byref(pt)))
+        return lanes.value, pt.value
+
+    def autotune(self, on=True):
+        """Time every launch shape on the next warm splits and keep the fastest."""
+        check(self.lib.v2g_set_autotune(self.h, int(bool(on))))
+
     # ---- path inputs (host) -------------------------------------------
     def BuildConstraints(self):
         b = np.empty(self.nconstraints)
@@ -249,16 +264,23 @@ class V2RDMSolver:
         check(self.lib.v2g_set_iter_callback(self.h, self._cb, None))
 
     def compute_energy(self, h1=None, eri=None, enuc=0.0, verbose=False, max_seconds=0.0,
-                       guess=True):
-        """BPSDP loop; returns a stats dict with energies (enuc added)."""
+                       guess=True, resume=False):
+        """BPSDP loop; returns a stats dict with energies (enuc added).
+
+        resume=True continues the previous call (or a loaded checkpoint) from
+        its x, z, y, mu, iteration count and errors instead of restarting the
+        iteration at mu = 1 (implies guess=False)."""
         if h1 is not None:
             self.GetIntegrals(h1, eri)
         if self.c is None:
             raise ValueError("no integrals: pass h1/eri or call GetIntegrals first")
         if self.b is None:
             self.BuildConstraints()
-        if guess or self.x is None:
+        if resume and self.x is None:
+            raise ValueError("resume needs x, z, y from a previous call or a checkpoint")
+        if (guess and not resume) or self.x is None:
             self.Guess()
+            self.last_stats = None
         o = self.options
         pc = {"NONE": 0, "JACOBI": 1, "LEARNED": 3}[str(o["PRECONDITIONER"]).upper()]
         p = _lib.Params(o["TAU_PARAMETER"], o["R_CONVERGENCE"], o["E_CONVERGENCE"],
@@ -268,6 +290,9 @@ class V2RDMSolver:
                         int(o["MLP_TRAIN_BATCH"]), int(o["MLP_TRAIN_STEPS"]),
                         float(o["MLP_TRAIN_LR"]), int(o["EIG_WARM_SWEEPS"]),
                         int(o["EIG_INNER_SWEEPS"]), int(bool(o["CG_EXTRAPOLATE"])))
+        prev = self.last_stats if resume else None
+        if prev:
+            p.mu0, p.oiter0, p.ep0, p.ed0 = prev["mu"], prev["oiter"], prev["ep"], prev["ed"]
         st = _lib.Stats()
         self._sync()
         check(self.lib.v2g_bpsdp(self.h, ctypes.byref(p), _ptr(self.c), _ptr(self.b),
@@ -275,4 +300,5 @@ class V2RDMSolver:
         out = {k: getattr(st, k) for k, _ in _lib.Stats._fields_}
         out["energy"] = out["energy_primal"] + enuc
         self.mu = out["mu"]
+        self.last_stats = out
         return out
