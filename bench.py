@@ -5,29 +5,40 @@ full active space, DQG + spin constraints, FP64 -- the configuration the
 headline metric names.  A step is one outer BPSDP iteration (rhs build, CG on
 A A^T, Update_xz eigen split, primal/dual errors), exactly as
 v2RDMSolver::compute_energy does it; `value` is outer iterations per second
-with every vector resident in HBM.
+of the plain-CG leg (the reference's CG) with every vector resident in HBM.
+
+Each leg runs W untimed outer iterations from the RANDOM guess, then K timed
+ones that continue the same solve (same mu, iteration count and errors: the
+timed call resumes the warm-up call, it does not restart the iteration).
 
 The Update_xz eigen split runs warm: the eigenvectors of the previous outer
 iteration seed one block-Jacobi sweep (one cyclic-Jacobi sweep per 64 x 64 pair
 sub-problem), which tracks the invariant subspaces from iteration to iteration;
-on H24 this reaches R/E_CONVERGENCE 1e-6 in the same number of outer
-iterations, at the same energy, as fully converged eigendecompositions.
+before the stopping rule accepts an iteration its split is redone to
+convergence, so a converged solve ends at the reference's fixed point.
 
 Also reported on the same JSON line:
   legs          the same timing for each CG variant (plain CG as in the
-                reference, Jacobi-preconditioned CG, learned preconditioner
-                trained during the warmup), each from the same RANDOM guess;
+                reference, Jacobi-preconditioned CG, learned preconditioner;
+                the learned leg trains for --train-iters outer iterations and
+                its warm-up covers them); a leg whose learned preconditioner
+                fell back to Jacobi is marked invalid;
   roofline      the A^T.y + A.x map kernels (the CG operator), timed with HIP
                 events on the engine's stream: algorithmic bytes per launch
                 8 (dimx + ncon) / mean launch time, against 8 TB/s HBM;
   eig           the Update_xz eigen split as the loop runs it (wall time per
-                outer iteration on the engine stream), against the algorithmic FP64
-                flops of a full eigendecomposition, sum (16/3) N^3, and 78.6 TFLOP/s;
-                one converged warm call, timed with HIP events, is reported beside it;
-  cpu_baseline  the CPU oracle (OpenMP maps + MKL dsyev) on the host, on a
-                bounded sample (one CG operator application and one Update_xz
-                timed; an outer iteration projected with the GPU run's mean CG
-                iterations per outer iteration).
+                outer iteration on the engine stream) and one converged warm
+                call (HIP events), each against the algorithmic FP64 flops of
+                a full eigendecomposition, sum (16/3) N^3, and 78.6 TFLOP/s;
+  peaks         HBM copy bandwidth and FP64 GEMM rate measured on this GPU
+                (device copy of 4 GiB, hipBLAS DGEMM 8192^3 through torch);
+  cpu_baseline  the CPU oracle (OpenMP maps, MKL dsyev) running the same
+                solve's outer iterations on the host, continued from the
+                GPU plain leg's state after its timed steps (--cpu-passes of
+                them), with the host's CPU model and core counts;
+  time_to_converge  the recorded runs to R/E_CONVERGENCE 1e-6 of
+                profiles/bpsdp_<workload>_converge.md (bench.py --converge),
+                when that file exists.
 """
 import argparse
 import json
@@ -83,25 +94,90 @@ def _eig_flops(dims):
     return sum((16.0 / 3.0) * float(n) ** 3 for n in dims)
 
 
-def cpu_baseline(w, gpu_cg_per_outer, positivity="DQG"):
-    """Time the oracle's CG operator and Update_xz on a bounded sample."""
+def host_info():
+    """CPU model, physical cores and logical CPUs of this host (lscpu)."""
+    import subprocess
+    info = dict(logical_cpus=os.cpu_count())
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = {k.strip(): v.strip() for k, v in
+              (line.split(":", 1) for line in out.splitlines() if ":" in line)}
+        info["cpu_model"] = kv.get("Model name")
+        cores = int(kv.get("Core(s) per socket", 0)) * int(kv.get("Socket(s)", 0))
+        info["physical_cores"] = cores or None
+    except Exception as e:  # reported, not fatal
+        info["lscpu_error"] = str(e)
+    return info
+
+
+def cpu_baseline(w, state, passes, positivity="DQG"):
+    """The oracle's own BPSDP loop (plain CG as in the reference, MKL dsyev in
+    Update_xz) on the host, continuing the GPU solve from `state` (x, z, y,
+    mu, oiter, ep, ed after the GPU's timed steps) for `passes` outer
+    iterations: the same iterations the GPU timed, on the CPU."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle_py as orc
     L = orc.Layout(w["amopi"], w["na"], w["nb"], None, positivity)
-    c = L.build_c(w["h1"], w["eri"])
-    x, z, y = L.guess_random()
-    t0 = time.time()
-    L.cg_Ax(y)
-    t_op = time.time() - t0
-    t0 = time.time()
-    L.update_xz(1.0, y, c, x)
-    t_xz = time.time() - t0
-    per_outer = t_xz + (gpu_cg_per_outer + 4.0) * t_op
-    return dict(value=1.0 / per_outer, unit="outer iter/s", cores=orc.num_threads(),
+    c, b = L.build_c(w["h1"], w["eri"]), L.build_b()
+    x, z, y = (np.ascontiguousarray(state[k]) for k in ("x", "z", "y"))
+    resume = {k: state[k] for k in ("mu", "oiter", "ep", "ed")}
+    t0 = time.perf_counter()
+    r = L.bpsdp(c, b, x, z, y, maxiter=state["oiter"] + passes, resume=resume)
+    dt = time.perf_counter() - t0
+    done = r["oiter"] - state["oiter"]
+    return dict(value=round(done / dt, 5), unit="outer iter/s", cores=orc.num_threads(),
                 kind="port", lapack="MKL dsyev" if orc.lapack_backend() else "built-in Jacobi",
-                sample=(f"oracle on the same workload: one A(A^T y) = {t_op:.3f} s and one "
-                        f"Update_xz = {t_xz:.3f} s timed; outer iteration projected as "
-                        f"Update_xz + ({gpu_cg_per_outer:.1f} CG + 4 map) applications"))
+                seconds_per_outer=round(dt / max(1, done), 3),
+                cg_iters_per_outer=round(r["cg_iters"] / max(1, done), 2),
+                t_cg_s=round(r["t_cg"], 3), t_update_xz_s=round(r["t_xz"], 3), **host_info(),
+                sample=(f"{done} outer iterations of the oracle's BPSDP loop (plain CG, MKL "
+                        f"dsyev) on the same workload, continuing the GPU plain-CG solve from "
+                        f"outer iteration {state['oiter']}; {dt:.1f} s"))
+
+
+def measure_peaks(dev):
+    """HBM copy bandwidth and FP64 GEMM rate on this GPU, outside any profiler:
+    a device-to-device copy of 4 GiB (read + write bytes counted) and an
+    8192^3 DGEMM through torch (hipBLAS), median of 5 timed repeats each."""
+    import torch
+    n = (4 << 30) // 8
+    src = torch.ones(n, dtype=torch.float64, device=dev)
+    dst = torch.empty_like(src)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+
+    def timed(fn, reps=5):
+        fn()
+        ts = []
+        for _ in range(reps):
+            ev[0].record()
+            fn()
+            ev[1].record()
+            ev[1].synchronize()
+            ts.append(ev[0].elapsed_time(ev[1]) / 1e3)
+        return sorted(ts)[len(ts) // 2]
+
+    t_copy = timed(lambda: dst.copy_(src))
+    del src, dst
+    m = 8192
+    a = torch.randn(m, m, dtype=torch.float64, device=dev)
+    bm = torch.randn(m, m, dtype=torch.float64, device=dev)
+    t_gemm = timed(lambda: torch.matmul(a, bm))
+    del a, bm
+    torch.cuda.empty_cache()
+    return dict(copy_gbs=round(2 * 8 * n / t_copy / 1e9, 1), dgemm_tflops=round(2 * m ** 3 / t_gemm / 1e12, 2),
+                spec_hbm_gbs=HBM_PEAK_GBS, spec_fp64_tflops=FP64_PEAK_TFLOPS,
+                how="device copy of 4 GiB (bytes read + written), torch.matmul FP64 8192^3, "
+                    "median of 5")
+
+
+def recorded_converge(workload):
+    """Runs to R/E_CONVERGENCE 1e-6 committed under profiles/ (one JSON record
+    per line starting with 'record ')."""
+    path = os.path.join(ROOT, "profiles", f"bpsdp_{workload}_converge.md")
+    if not os.path.exists(path):
+        return None
+    recs = [json.loads(line[len("record "):]) for line in open(path) if line.startswith("record ")]
+    return dict(source=os.path.relpath(path, ROOT), runs=recs) if recs else None
 
 
 def _maps_only(s, stream, reps, w):
@@ -140,7 +216,7 @@ def _scan(args):
     from v2rdm_casscf_amd.bondscan import make_hchain_scan, solve_scan
     pts = make_hchain_scan(args.scan, seed=0)
     opts = dict(PRECONDITIONER=args.precond.upper(), EIG_WARM_SWEEPS=args.eig_sweeps,
-                EIG_INNER_SWEEPS=args.inner_sweeps, MAXITER=10 ** 6)
+                EIG_INNER_SWEEPS=args.inner_sweeps, EIG_PAIR_ORDER=args.eig_order, MAXITER=10 ** 6)
 
     def progress(i, r):
         if i < 0:
@@ -172,11 +248,20 @@ def main():
     ap.add_argument("--legs", default="none,jacobi,learned",
                     help="CG variants timed side by side (same guess, warmup and steps each)")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-peaks", action="store_true")
+    ap.add_argument("--cpu-passes", type=int, default=2,
+                    help="outer iterations of the CPU oracle, continuing the GPU solve")
+    ap.add_argument("--train-iters", type=int, default=30,
+                    help="outer iterations of online training of the learned preconditioner "
+                         "(the learned leg's warm-up covers them)")
     ap.add_argument("--map-reps", type=int, default=20)
     ap.add_argument("--eig-sweeps", type=int, default=1,
                     help="block-Jacobi sweeps per warm Update_xz inside the loop (0: converge); "
                          "one sweep tracks the eigenvectors from iteration to iteration with the "
                          "same outer convergence as fully converged splits")
+    ap.add_argument("--eig-order", type=int, default=1,
+                    help="pair order of a tracking sweep: 1 cyclic in its first round, then only "
+                         "the cross-block pairs (32 steps); 0 cyclic in every round (63 steps)")
     ap.add_argument("--inner-sweeps", type=int, default=1,
                     help="cyclic-Jacobi sweeps per 64x64 pair sub-problem of a warm call (0: 12)")
     ap.add_argument("--positivity", default="DQG", help="D, DQ, DG, DQG, DQGT1, DQGT2, DQGT")
@@ -209,7 +294,8 @@ def main():
     s = V2RDMSolver(w["amopi"], w["na"], w["nb"], 1, args.positivity, device=dev,
                     options=dict(PRECONDITIONER=args.precond.upper(),
                                  EIG_WARM_SWEEPS=args.eig_sweeps,
-                                 EIG_INNER_SWEEPS=args.inner_sweeps))
+                                 EIG_INNER_SWEEPS=args.inner_sweeps,
+                                 EIG_PAIR_ORDER=args.eig_order))
     stream = torch.cuda.Stream(device=dev)
     s.lib.v2g_set_stream(s.h, __import__("ctypes").c_void_p(stream.cuda_stream))
     s.GetIntegrals(w["h1"], w["eri"])
@@ -221,6 +307,7 @@ def main():
         return
     if args.converge > 0:
         s.options["MAXITER"] = 10 ** 7
+        s.options["MLP_TRAIN_ITERS"] = args.train_iters
         t_last = [time.perf_counter()]
 
         def progress(oiter, st):
@@ -237,7 +324,8 @@ def main():
         torch.cuda.synchronize()
         secs = time.perf_counter() - t0
         keep = ("oiter", "cg_iters", "converged", "ep", "ed", "mu", "t_cg", "t_xz",
-                "precond_active", "fallback", "cg_iters_train", "train_outer", "mlp_loss")
+                "precond_active", "fallback", "cg_iters_train", "train_outer", "mlp_loss",
+                "exact_splits")
         rec = dict(mode="time-to-converge", workload=w["name"], positivity=args.positivity,
                    precond=args.precond, dimx=s.dimx, ncon=s.nconstraints, seconds=round(secs, 3),
                    energy=st["energy_primal"] + w["enuc"], energy_dual=st["energy_dual"] + w["enuc"],
@@ -260,43 +348,59 @@ def main():
         print(json.dumps(rec))
         return
 
-    def timed_leg(solver):
-        """W untimed outer iterations from the RANDOM guess, then K timed ones."""
-        solver.options["MAXITER"] = args.warmup
+    def timed_leg(solver, warmup):
+        """`warmup` untimed outer iterations from the RANDOM guess, then K timed
+        ones continuing the same solve (resume: no restart of mu or the count)."""
+        solver.options["MAXITER"] = warmup
         st_w = solver.compute_energy(guess=False)
-        solver.options["MAXITER"] = args.steps
+        solver.options["MAXITER"] = st_w["oiter"] + args.steps
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        st = solver.compute_energy(guess=False)
+        st = solver.compute_energy(resume=True)
         torch.cuda.synchronize()
         return st_w, st, time.perf_counter() - t0
 
+    def leg_record(st_w, st, el):
+        n = st["oiter"] - st_w["oiter"]
+        rec = dict(value=round(n / el, 4), ms_per_step=round(1e3 * el / n, 3), steps=n,
+                   warmup=st_w["oiter"], cg_iters_per_outer=round(st["cg_iters"] / max(1, n), 2),
+                   ep=st["ep"], ed=st["ed"], fallback=st.get("fallback", 0),
+                   precond_active=st.get("precond_active", 0))
+        if st.get("fallback", 0):
+            # the learned preconditioner lost its A/B test and the leg ran
+            # Jacobi-CG: not a measurement of the learned method
+            rec["valid"] = False
+            rec["value"] = None
+        return rec
+
+    def leg_warmup(leg):
+        return max(args.warmup, args.train_iters + 2) if leg == "learned" else args.warmup
+
     legs = {}
     for leg in [l for l in args.legs.split(",") if l and l != args.precond]:
-        o = dict(s.options, PRECONDITIONER=leg.upper(), MLP_TRAIN_ITERS=max(1, args.warmup - 2))
+        o = dict(s.options, PRECONDITIONER=leg.upper(), MLP_TRAIN_ITERS=args.train_iters)
         sl = V2RDMSolver(w["amopi"], w["na"], w["nb"], 1, args.positivity, device=dev, options=o)
         sl.GetIntegrals(w["h1"], w["eri"])
         sl.BuildConstraints()
         sl.Guess()
-        _, stl, el = timed_leg(sl)
-        legs[leg] = dict(value=round(stl["oiter"] / el, 4), ms_per_step=round(1e3 * el / stl["oiter"], 3),
-                         cg_iters_per_outer=round(stl["cg_iters"] / max(1, stl["oiter"]), 2),
-                         ep=stl["ep"], ed=stl["ed"], fallback=stl.get("fallback", 0),
-                         precond_active=stl.get("precond_active", 0))
+        legs[leg] = leg_record(*timed_leg(sl, leg_warmup(leg)))
+        print(f"# leg {leg}: {legs[leg]}", file=sys.stderr, flush=True)
         del sl
         torch.cuda.empty_cache()
 
-    if args.precond == "learned":
-        s.options["MLP_TRAIN_ITERS"] = max(1, args.warmup - 2)
-    st_w, st, elapsed = timed_leg(s)
-    nsteps = st["oiter"]
+    s.options["MLP_TRAIN_ITERS"] = args.train_iters
+    st_w, st, elapsed = timed_leg(s, leg_warmup(args.precond))
+    nsteps = st["oiter"] - st_w["oiter"]
     ms_per_step = 1e3 * elapsed / nsteps
-    legs[args.precond] = dict(value=round(nsteps / elapsed, 4), ms_per_step=round(ms_per_step, 3),
-                              cg_iters_per_outer=round(st["cg_iters"] / max(1, nsteps), 2),
-                              ep=st["ep"], ed=st["ed"], fallback=st.get("fallback", 0),
-                              precond_active=st.get("precond_active", 0))
+    legs[args.precond] = leg_record(st_w, st, elapsed)
+    print(f"# leg {args.precond}: {legs[args.precond]}", file=sys.stderr, flush=True)
+    # the solve's state after the timed steps, for the CPU baseline
+    state = dict(x=s.x.cpu().numpy(), z=s.z.cpu().numpy(), y=s.y.cpu().numpy(), mu=st["mu"],
+                 oiter=st["oiter"], ep=st["ep"], ed=st["ed"])
 
     # per-kernel timing with HIP events on the engine's stream
+    import ctypes
+    vp = ctypes.c_void_p
     ncon, dimx = s.nconstraints, s.dimx
     yv = s.y.clone()
     out = torch.empty(ncon, dtype=torch.float64, device=dev)
@@ -306,16 +410,14 @@ def main():
         s.bpsdp_ATu(tmp, yv)
         ev0.record(stream)
         for _ in range(args.map_reps):
-            s.lib.v2g_ATu(s.h, __import__("ctypes").c_void_p(yv.data_ptr()),
-                          __import__("ctypes").c_void_p(tmp.data_ptr()))
+            s.lib.v2g_ATu(s.h, vp(yv.data_ptr()), vp(tmp.data_ptr()))
         ev1.record(stream)
     ev1.synchronize()
     t_atu = ev0.elapsed_time(ev1) / 1e3 / args.map_reps
     with torch.cuda.stream(stream):
         ev0.record(stream)
         for _ in range(args.map_reps):
-            s.lib.v2g_Au(s.h, __import__("ctypes").c_void_p(tmp.data_ptr()),
-                         __import__("ctypes").c_void_p(out.data_ptr()))
+            s.lib.v2g_Au(s.h, vp(tmp.data_ptr()), vp(out.data_ptr()))
         ev1.record(stream)
     ev1.synchronize()
     t_au = ev0.elapsed_time(ev1) / 1e3 / args.map_reps
@@ -323,14 +425,12 @@ def main():
     xs, zs = s.x.clone(), s.z.clone()
     with torch.cuda.stream(stream):
         ev0.record(stream)
-        s.lib.v2g_update_xz(s.h, __import__("ctypes").c_double(st["mu"]),
-                            __import__("ctypes").c_void_p(yv.data_ptr()),
-                            __import__("ctypes").c_void_p(s.c.data_ptr()),
-                            __import__("ctypes").c_void_p(xs.data_ptr()),
-                            __import__("ctypes").c_void_p(zs.data_ptr()))
+        s.lib.v2g_update_xz(s.h, ctypes.c_double(st["mu"]), vp(yv.data_ptr()), vp(s.c.data_ptr()),
+                            vp(xs.data_ptr()), vp(zs.data_ptr()))
         ev1.record(stream)
     ev1.synchronize()
     t_xz = ev0.elapsed_time(ev1) / 1e3
+    del xs, zs, out, tmp, yv
     eflops = _eig_flops([n for n in s.dimensions if n > 0])
 
     cg_per_outer = st["cg_iters"] / max(1, nsteps)
@@ -343,25 +443,34 @@ def main():
     if tr and "k_atu" in tr and "k_au" in tr:
         roofline["traffic"] = int((tr["k_atu"] + tr["k_au"]) / 2)
         roofline["traffic_k_atu"], roofline["traffic_k_au"] = tr["k_atu"], tr["k_au"]
-    # the eigen split as the loop runs it (warm, tracked): its wall time per outer
-    # iteration on the engine stream, against the flops of a full eigendecomposition;
-    # the converged warm call measured with events above is reported beside it
+    # the eigen split as the loop runs it (warm, tracked: one sweep) and one
+    # converged warm call, each against the flops of a full eigendecomposition
     t_xz_loop = st["t_xz"] / max(1, nsteps)
     eig = dict(bound="mfma", achieved=round(eflops / t_xz_loop / 1e12, 3), peak=FP64_PEAK_TFLOPS,
                unit="TFLOP/s", t_update_xz_loop_ms=round(1e3 * t_xz_loop, 3), flops=eflops,
                warm_sweeps_per_call=args.eig_sweeps, inner_sweeps=args.inner_sweeps,
-               t_update_xz_converged_ms=round(1e3 * t_xz, 3))
+               pair_order=args.eig_order,
+               t_update_xz_converged_ms=round(1e3 * t_xz, 3),
+               achieved_converged=round(eflops / t_xz / 1e12, 3),
+               frac_converged=round(eflops / t_xz / 1e12 / FP64_PEAK_TFLOPS, 4),
+               launch_shape=dict(zip(("lanes", "pair_threads"), s.eig_shape())))
     eig["frac"] = round(eig["achieved"] / FP64_PEAK_TFLOPS, 4)
+    peaks = None
+    if not args.no_peaks:
+        try:
+            peaks = measure_peaks(dev)
+        except Exception as e:  # reported, not fatal
+            peaks = dict(error=str(e))
     cpu = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(w, cg_per_outer, args.positivity)
+            cpu = cpu_baseline(w, state, args.cpu_passes, args.positivity)
         except Exception as e:  # reported, not fatal
             cpu = dict(error=str(e))
     value = nsteps / elapsed
     line = dict(metric=METRIC,
                 value=round(value, 4), unit="outer iter/s", n_gpus=1, steps=nsteps,
-                warmup=args.warmup, ms_per_step=round(ms_per_step, 3), higher_is_better=True,
+                warmup=st_w["oiter"], ms_per_step=round(ms_per_step, 3), higher_is_better=True,
                 scaling="weak", vs_baseline=None, dtype="f64", data="synthetic",
                 config=dict(workload=w["name"], dimx=dimx, ncon=ncon,
                             precond=args.precond, cg_iters_per_outer=round(cg_per_outer, 2),
@@ -369,7 +478,8 @@ def main():
                             mu=st["mu"], t_cg_s=st["t_cg"], t_xz_s=st["t_xz"],
                             eig_warm_sweeps=args.eig_sweeps, eig_inner_sweeps=args.inner_sweeps,
                             warmup_s=st_w["t_total"]),
-                legs=legs, roofline=roofline, eig=eig, cpu_baseline=cpu)
+                legs=legs, roofline=roofline, eig=eig, peaks=peaks, cpu_baseline=cpu,
+                time_to_converge=recorded_converge(args.workload))
     print(json.dumps(line))
 
 

@@ -89,7 +89,13 @@ int v2g_set_autotune(v2g_handle h, int on);
 /* ---- learned preconditioner (no reference counterpart) ----
  * A per-row MLP (14 inputs: one-hot constraint family x11, log diag(A A^T),
  * asinh(rhs/rms), asinh(y_prev/rms); tanh hidden layers 64, 64; one output g)
- * gives M^-1_ii = exp(clamp(g_i, -3, 3)) / diag(A A^T)_ii.  Weights are FP32,
+ * gives M^-1_ii = exp(clamp(g_i, -3, 3)), divided by diag(A A^T)_ii with the
+ * Jacobi base (v2g_params.mlp_base = 1).  Inside v2g_bpsdp (precond 3) it is
+ * trained online during the first train_iters outer iterations, whose CG
+ * solves use the base method, on the spectral objective log rho(r_hi) -
+ * log rho(r_lo) (rho the Rayleigh quotient of the preconditioned operator;
+ * r_lo the base solve's final residual, r_hi from power iterations), then
+ * used in flexible CG, guarded by the true residual.  Weights are FP32,
  * packed W1[14][64], b1[64], W2[64][64], b2[64], w3[64], b3 (5185 floats). */
 int v2g_mlp_num_params(void);
 /* seeded init; the last layer starts at zero, i.e. exactly Jacobi */
@@ -97,9 +103,17 @@ int v2g_mlp_init(v2g_handle h, unsigned long long seed);
 int v2g_mlp_get_weights(v2g_handle h, float *w_host);
 int v2g_mlp_set_weights(v2g_handle h, const float *w_host);
 /* loss = sum_b wt_b (out_b - tgt_b)^2 / nb and its gradient (FP64 out,
- * device pointers; feat is nb x 14 FP32) -- the training kernels */
+ * device pointers; feat is nb x 14 FP32) -- the training kernels; with
+ * tgt_dev = NULL, wt_dev holds the upstream gradient d(loss)/d(out_b) per
+ * sample (the form online training uses) and loss is 0 */
 int v2g_mlp_loss_grad(v2g_handle h, int nb, const float *feat_dev, const float *tgt_dev,
                       const float *wt_dev, double *grad_dev, double *loss_host);
+/* the training signal of the spectral objective for every row: dout_dev[i]
+ * (FP32) = d L / d g_i, L = log rho(r_hi) - log rho(r_lo) with S the current
+ * learned diagonal from (rhs, y); objective receives L */
+int v2g_mlp_spectral_grad(v2g_handle h, const double *rhs_dev, const double *y_dev,
+                          const double *rhi_dev, const double *rlo_dev, float *dout_dev,
+                          double *objective);
 /* one Adam step (lr) on the current weights with grad_dev */
 int v2g_mlp_adam_step(v2g_handle h, const double *grad_dev, double lr);
 /* inference: minv for every constraint row from rhs and y (device) */
@@ -136,6 +150,14 @@ typedef struct v2g_params {
     int oiter0;                  /* outer iterations already done          */
     double ep0, ed0;             /* their last primal / dual errors (CG    */
                                  /* tolerance of the first iteration)      */
+    int mlp_base;                /* precond 3: 0 the learned diagonal      */
+                                 /* scales the identity (plain CG), 1 the  */
+                                 /* Jacobi diagonal                        */
+    int eig_pair_order;          /* capped warm splits: 0 every round's    */
+                                 /* 64 x 64 pairs in cyclic order; 1 cyclic */
+                                 /* in the first round of a sweep, then    */
+                                 /* only the pairs across the two column   */
+                                 /* blocks (32 instead of 63 steps)        */
 } v2g_params;
 
 typedef struct v2g_stats {
@@ -177,6 +199,28 @@ int v2g_build_c(v2g_handle h, const double *h1_host, const double *eri_host, dou
 /* TPDM_GUESS RANDOM (v2rdm_solver.cc:2479-2490): srand(0), then rand() for
  * x, z (interleaved) and y, each in [-2, 0] */
 int v2g_guess_random(v2g_handle h, double *x_host, double *z_host, double *y_host);
+
+/* The same three inputs built on the device, for batched runs (no reference
+ * counterpart beyond the functions above; results equal theirs bit for bit):
+ * b (ncon) and c (dimx) from device h1 (amo x amo) and eri (amo^4); the guess
+ * from glibc's rand() sequence after srand(0), restated on the device. */
+int v2g_build_b_device(v2g_handle h, double *b_dev);
+int v2g_build_c_device(v2g_handle h, const double *h1_dev, const double *eri_dev, double *c_dev);
+int v2g_guess_random_device(v2g_handle h, double *x_dev, double *z_dev, double *y_dev);
+
+/* ---- RDM outputs (x on the device, results in host memory) ----
+ * v2RDMSolver::get_opdm (python_api.cc:376-396): per irrep h an amopi[h] x
+ * amopi[h] block, blocks concatenated in irrep order; the spin-summed
+ * D1a + D1b (the reference adds the D1a block twice, which is the same for
+ * the spin-constrained singlets it is used with). */
+int v2g_get_opdm(v2g_handle h, const double *x_dev, double *opdm_host);
+/* v2RDMSolver::get_tpdm (python_api.cc:398-458): amo^2 x amo^2, element
+ * [i amo + k][j amo + l]; C1 only (-1 with the reference's message otherwise) */
+int v2g_get_tpdm(v2g_handle h, const double *x_dev, double *tpdm_host);
+/* the RDM file of v2RDMSolver::FCIDUMP (fcidump.cc:176-268): binary records
+ * (double value, int i, j, k, l) of the spin-summed 2-RDM, then the 1-RDM
+ * with k = l = 0; active orbitals numbered from 1 in Pitzer order (no core) */
+int v2g_write_fcidump_rdm(v2g_handle h, const double *x_dev, const char *path);
 
 const char *v2g_last_error(void);
 const char *v2g_version(void);

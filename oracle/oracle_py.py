@@ -37,7 +37,9 @@ class _Params(ctypes.Structure):
     _fields_ = [("tau", ctypes.c_double), ("r_convergence", ctypes.c_double),
                 ("e_convergence", ctypes.c_double), ("cg_convergence", ctypes.c_double),
                 ("maxiter", ctypes.c_int), ("cg_maxiter", ctypes.c_int),
-                ("mu_update_frequency", ctypes.c_int), ("verbose", ctypes.c_int)]
+                ("mu_update_frequency", ctypes.c_int), ("verbose", ctypes.c_int),
+                ("mu0", ctypes.c_double), ("oiter0", ctypes.c_int), ("ep0", ctypes.c_double),
+                ("ed0", ctypes.c_double)]
 
 
 class _Stats(ctypes.Structure):
@@ -238,9 +240,15 @@ class Layout:
 
     def bpsdp(self, c, b, x, z, y, r_conv=1e-6, e_conv=1e-6, cg_conv=1e-9, maxiter=10000,
               cg_maxiter=10000, tau=1.0, mu_update_frequency=500, verbose=False,
-              max_seconds=0.0):
+              max_seconds=0.0, resume=None):
+        """The reference loop; resume = dict(mu, oiter, ep, ed) of an earlier
+        run continues it (x, z, y passed in hold that run's state), and maxiter
+        counts from the start of the whole solve, as in the reference."""
         p = _Params(tau, r_conv, e_conv, cg_conv, maxiter, cg_maxiter, mu_update_frequency,
                     int(verbose))
+        if resume:
+            p.mu0, p.oiter0 = float(resume["mu"]), int(resume["oiter"])
+            p.ep0, p.ed0 = float(resume["ep"]), float(resume["ed"])
         st = _Stats()
         lib().orc_bpsdp(self.h, ctypes.byref(p), _d(c), _d(b), _d(x), _d(z), _d(y),
                         float(max_seconds), ctypes.byref(st))
@@ -278,6 +286,7 @@ def mlp_loss_grad(feat, target, wt, W1, b1, W2, b2, w3, b3):
     nh = W1.shape[1]
     g = [np.zeros(nin * nh), np.zeros(nh), np.zeros(nh * nh), np.zeros(nh), np.zeros(nh),
          np.zeros(1)]
-    loss = lib().orc_mlp_loss_grad(n, nin, nh, _f(feat), _f(target), _f(wt), _f(W1), _f(b1),
+    tp = _f(target) if target is not None else ctypes.POINTER(ctypes.c_float)()
+    loss = lib().orc_mlp_loss_grad(n, nin, nh, _f(feat), tp, _f(wt), _f(W1), _f(b1),
                                    _f(W2), _f(b2), _f(w3), float(b3), *[_d(a) for a in g])
     return loss, g

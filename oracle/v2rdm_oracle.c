@@ -703,9 +703,10 @@ int orc_bpsdp(const orc_layout *L, const orc_params *p, const double *c, const d
     double *Ax = (double *)malloc(sizeof(double) * ncon);
     double *B = (double *)malloc(sizeof(double) * ncon);
     double *ATy = (double *)malloc(sizeof(double) * dimx);
-    double tau = p->tau, mu = 1.0;
-    double ep = 0.0, ed = 0.0, egap = 0.0, eprimal = 0.0, edual = 0.0;
-    int oiter = 0, converged = 0;
+    double tau = p->tau, mu = p->mu0 > 0.0 ? p->mu0 : 1.0;
+    double ep = p->ep0, ed = p->ed0, egap = 0.0, eprimal = 0.0, edual = 0.0;
+    int oiter = p->oiter0 > 0 ? p->oiter0 : 0, converged = 0;
+    const int oiter_first = oiter;
     long cgtot = 0;
     double tcg = 0.0, txz = 0.0, t0 = wall();
     do {
@@ -719,7 +720,7 @@ int orc_bpsdp(const orc_layout *L, const orc_params *p, const double *c, const d
 #pragma omp parallel for schedule(static)
         for (long i = 0; i < ncon; i++) B[i] += Ax[i];        /* :1794 */
         double cgtol;                                         /* :1797-1805 */
-        if (oiter == 0) cgtol = 0.01;
+        if (oiter == 0 || (oiter == oiter_first && !(ep > 0.0 && ed > 0.0))) cgtol = 0.01;
         else cgtol = (ep > ed) ? 0.01 * ed : 0.01 * ep;
         if (cgtol < p->cg_convergence) cgtol = p->cg_convergence;
         int iiter = orc_cg_solve(L, B, y, cgtol, p->cg_maxiter, Ax);  /* :1808 */
@@ -745,7 +746,7 @@ int orc_bpsdp(const orc_layout *L, const orc_params *p, const double *c, const d
         oiter++;
         egap = fabs(ecur - edual);
         eprimal = ecur;
-        if (oiter == p->maxiter) break;
+        if (oiter >= p->maxiter) break;
         if (max_seconds > 0 && wall() - t0 > max_seconds) break;
     } while (ep > p->r_convergence || ed > p->r_convergence || egap > p->e_convergence);
     converged = !(ep > p->r_convergence || ed > p->r_convergence || egap > p->e_convergence);
@@ -813,10 +814,15 @@ double orc_mlp_loss_grad(int n, int nin, int nh, const float *feat, const float 
         }
         double o = b3;
         for (int j = 0; j < nh; j++) o += (double)w3[j] * h2[j];
-        double r = o - target[e];
-        double w = wt ? wt[e] : 1.0;
-        loss += w * r * r;
-        double go = 2.0 * w * r / n;
+        double go;
+        if (target) {
+            double r = o - target[e];
+            double w = wt ? wt[e] : 1.0;
+            loss += w * r * r;
+            go = 2.0 * w * r / n;
+        } else {
+            go = wt[e];   /* upstream gradient d(loss)/d(out) given per sample */
+        }
         *gb3 += go;
         for (int j = 0; j < nh; j++) {
             gw3[j] += go * h2[j];

@@ -105,6 +105,11 @@ typedef struct orc_params {
     double tau, r_convergence, e_convergence, cg_convergence;
     int maxiter, cg_maxiter, mu_update_frequency;
     int verbose;
+    /* continuation of an earlier run (all zero: a fresh solve from mu = 1):
+     * mu, outer iterations done and the last primal / dual errors */
+    double mu0;
+    int oiter0;
+    double ep0, ed0;
 } orc_params;
 
 typedef struct orc_stats {
@@ -130,7 +135,9 @@ int orc_num_threads(void);
 void orc_mlp_forward(int n, int nin, int nh, const float *feat, const float *W1,
                      const float *b1, const float *W2, const float *b2,
                      const float *w3, float b3, float *out);
-/* loss = sum_i wt_i (t_i - s_i)^2 / n with s_i = out_i; gradients of weights */
+/* loss = sum_i wt_i (t_i - s_i)^2 / n with s_i = out_i; gradients of weights;
+ * target = NULL: wt_i is the upstream gradient d(loss)/d(out_i) itself (the
+ * form the online training feeds the backward pass) and the loss is 0 */
 double orc_mlp_loss_grad(int n, int nin, int nh, const float *feat, const float *target,
                          const float *wt, const float *W1, const float *b1,
                          const float *W2, const float *b2, const float *w3, float b3,

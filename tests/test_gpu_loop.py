@@ -62,9 +62,14 @@ def test_loop_energy_matches_oracle(name, shape):
           f"(oracle {g['oiter']}), exact re-splits {out['exact_splits']}", flush=True)
     assert out["converged"]
     assert abs(out["energy"] - g["energy"]) < 1e-6
-    # the accepted iteration's split was redone to convergence
-    assert out["exact_splits"] >= 1
     assert out["ep"] < 1e-6 and out["ed"] < 1e-6
+    if max(s.dimensions) > 64:
+        # blocks beyond 64 are tracked with one sweep per iteration: the
+        # accepted iteration's split was redone to convergence
+        assert out["exact_splits"] >= 1
+    else:
+        # every block is solved whole (to convergence) in every iteration
+        assert out["exact_splits"] == 0
 
 
 SHAPES = [(1, 256), (1, 1024), (2, 256), (2, 1024)]

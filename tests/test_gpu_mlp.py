@@ -1,12 +1,14 @@
 """The learned preconditioner's HIP kernels against the CPU oracle, and what
-the trained network gains over Jacobi-CG.
+the trained network gains over the methods without it.
 
 Gradients: FP32 weights and activations on the device, FP64 in the oracle;
 tolerance 2e-4 relative per parameter block (FP32 accumulation over the
-batch).  Gain: the same BPSDP problem solved with Jacobi-CG and with the
+batch), for both the regression loss and the upstream-gradient form that the
+online training uses.  The spectral objective's per-row training signal is
+checked against central finite differences of the objective (dense A A^T).
+Gain: the same BPSDP problem solved with plain CG, Jacobi-CG and the
 online-trained learned preconditioner (same guess, same stopping rule);
-energies must agree to 1e-6 Eh and the CG iteration counts after the
-training phase are reported.
+energies must agree to 1e-6 Eh, and the CG work is compared.
 """
 import json
 
@@ -58,7 +60,60 @@ def test_mlp_grad_matches_oracle():
     assert loss2 < loss
 
 
-def test_mlp_forward_matches_oracle_and_starts_as_jacobi():
+def test_mlp_upstream_backprop_matches_oracle():
+    """The backward pass as online training runs it: no target, the upstream
+    gradient d(loss)/d(out) given per sample (tgt == NULL)."""
+    from v2rdm_casscf_amd import V2RDMSolver
+    s = V2RDMSolver([4], 2, 2, 1, "DQG")
+    rng = np.random.default_rng(13)
+    w = (rng.normal(size=s.mlp_num_params()) * 0.3).astype(np.float32)
+    s.mlp_set_weights(w)
+    nb = 517
+    feat = rng.normal(size=(nb, NIN)).astype(np.float32)
+    up = rng.normal(size=nb).astype(np.float32)
+    _, g = s.mlp_loss_grad(torch.from_numpy(feat).cuda(), None, torch.from_numpy(up).cuda())
+    W1, b1, W2, b2, w3, b3 = _split(w)
+    _, gref = orc.mlp_loss_grad(feat, None, up, W1.reshape(NIN, NH).copy(), b1.copy(),
+                                W2.reshape(NH, NH).copy(), b2.copy(), w3.copy(), b3[0])
+    for k, (a, b) in enumerate(zip(_split(g.cpu().numpy()), gref)):
+        assert np.linalg.norm(a - b) <= 2e-4 * np.linalg.norm(b) + 1e-9, k
+
+
+def test_spectral_signal_matches_finite_differences():
+    """d L / d g_i of L = log rho(r_hi) - log rho(r_lo), rho(r) =
+    (Sr).K(Sr) / (r.Sr), S = diag(exp(g)): the kernel's per-row signal
+    against central differences of L computed with a dense K = A A^T."""
+    from v2rdm_casscf_amd import V2RDMSolver
+    amopi, na = [3], 1
+    s = V2RDMSolver(amopi, na, na, 1, "DQG")
+    L = orc.Layout(amopi, na, na, 1, "DQG")
+    A = np.stack([L.Au(e) for e in np.eye(L.dimx)], axis=1)
+    K = A @ A.T
+    rng = np.random.default_rng(14)
+    w = (rng.normal(size=s.mlp_num_params()) * 0.2).astype(np.float32)
+    s.mlp_set_weights(w)
+    n = L.ncon
+    rhs, y = rng.normal(size=n), rng.normal(size=n)
+    rhi, rlo = rng.normal(size=n), rng.normal(size=n)
+    dev = lambda a: torch.from_numpy(a).cuda()
+    sig, obj = s.mlp_spectral_grad(dev(rhs), dev(y), dev(rhi), dev(rlo))
+    sv = s.mlp_minv(dev(rhs), dev(y)).cpu().numpy()      # S = exp(g), identity base
+    g0 = np.log(sv)
+    assert np.abs(g0).max() < 3.0 - 1e-3                  # away from the clamp
+
+    def lobj(g):
+        S = np.exp(g)
+        rho = lambda r: (S * r) @ K @ (S * r) / (r @ (S * r))
+        return np.log(rho(rhi)) - np.log(rho(rlo))
+
+    assert abs(obj - lobj(g0)) < 1e-9 * max(1.0, abs(obj))
+    h = 1e-6
+    fd = np.array([(lobj(g0 + h * e) - lobj(g0 - h * e)) / (2 * h) for e in np.eye(n)])
+    got = sig.cpu().numpy().astype(np.float64)
+    assert np.linalg.norm(got - fd) <= 1e-5 * np.linalg.norm(fd)
+
+
+def test_mlp_forward_matches_oracle_and_starts_as_its_base():
     from v2rdm_casscf_amd import V2RDMSolver
     s = V2RDMSolver([5], 2, 2, 1, "DQG")
     s.mlp_init(7)
@@ -67,15 +122,15 @@ def test_mlp_forward_matches_oracle_and_starts_as_jacobi():
     y = torch.from_numpy(rng.normal(size=s.nconstraints)).cuda()
     d = s.row_norm2()
     minv = s.mlp_minv(rhs, y)
-    # last layer initialised to zero: exactly the Jacobi diagonal
-    assert torch.allclose(minv, 1.0 / d, rtol=1e-6)
-    # random weights: positive, and equal to exp(clamp(oracle forward)) / d
+    # last layer initialised to zero: exactly the identity base (plain CG)
+    assert torch.equal(minv, torch.ones_like(minv))
+    # random weights: positive, and equal to exp(clamp(oracle forward))
     w = (rng.normal(size=s.mlp_num_params()) * 0.3).astype(np.float32)
     s.mlp_set_weights(w)
     minv = s.mlp_minv(rhs, y).cpu().numpy()
     assert (minv > 0).all()
     dn = d.cpu().numpy()
-    g = np.log(minv * dn)
+    g = np.log(minv)
     assert np.abs(g).max() <= 3.0 + 1e-6
     # rebuild the features the way the kernel does and compare to the oracle
     L = orc.Layout([5], 2, 2, 1, "DQG")
@@ -100,29 +155,40 @@ def test_mlp_forward_matches_oracle_and_starts_as_jacobi():
     assert np.allclose(g, np.clip(out, -3, 3), atol=2e-4)
 
 
-def test_learned_preconditioner_gain_over_jacobi():
+@pytest.mark.parametrize("n", [8, 12])
+def test_learned_preconditioner_gain(n):
+    """H_n to R/E_CONVERGENCE 1e-6 with plain CG (the reference), Jacobi-CG
+    and the learned diagonal (identity base, trained online on the first 30
+    outer iterations' plain-CG solves): same energy, and the learned path's
+    CG work is compared with the best method without it."""
     from v2rdm_casscf_amd import V2RDMSolver
     from v2rdm_casscf_amd.hamiltonians import hydrogen_chain
-    d = hydrogen_chain(8, 1.8)
+    d = hydrogen_chain(n, 1.8)
     res = {}
-    for pc in ("JACOBI", "LEARNED"):
-        s = V2RDMSolver(d["amopi"], 4, 4, 1, "DQG",
-                        options=dict(PRECONDITIONER=pc, MLP_TRAIN_ITERS=25,
-                                     R_CONVERGENCE=1e-6, E_CONVERGENCE=1e-6))
+    for pc in ("NONE", "JACOBI", "LEARNED"):
+        s = V2RDMSolver(d["amopi"], n // 2, n // 2, 1, "DQG",
+                        options=dict(PRECONDITIONER=pc, MLP_TRAIN_ITERS=30, MAXITER=100000))
         res[pc] = s.compute_energy(d["h1"], d["eri"], enuc=d["enuc"])
-    j, l = res["JACOBI"], res["LEARNED"]
-    assert j["converged"] and l["converged"]
-    assert abs(j["energy"] - l["energy"]) < 1e-6
-    # CG work after the training phase (the learned path trains during its
-    # first iterations with Jacobi-CG)
-    jt = j["cg_iters"]
-    lt = l["cg_iters"]
-    report = dict(workload="H8 STO-3G 1.8 bohr D2h DQG", jacobi_cg_iters=jt,
-                  learned_cg_iters=lt, jacobi_outer=j["oiter"], learned_outer=l["oiter"],
+    for pc, r in res.items():
+        assert r["converged"], pc
+        assert abs(r["energy"] - res["NONE"]["energy"]) < 1e-6, pc
+    best = min(("NONE", "JACOBI"), key=lambda k: res[k]["cg_iters"])
+    l = res["LEARNED"]
+    report = dict(workload=f"H{n} STO-3G 1.8 bohr D2h DQG",
+                  **{f"{k.lower()}_cg_iters": r["cg_iters"] for k, r in res.items()},
+                  **{f"{k.lower()}_outer": r["oiter"] for k, r in res.items()},
+                  **{f"{k.lower()}_seconds": round(r["t_total"], 3) for k, r in res.items()},
                   learned_train_outer=l["train_outer"], fallback=l["fallback"],
-                  jacobi_seconds=j["t_total"], learned_seconds=l["t_total"],
-                  cg_iteration_ratio=jt / max(1, lt), mlp_loss=l["mlp_loss"])
+                  best_without=best, objective_change=l["mlp_loss"],
+                  cg_iteration_ratio_vs_best=res[best]["cg_iters"] / max(1, l["cg_iters"]),
+                  cg_iteration_ratio_vs_jacobi=res["JACOBI"]["cg_iters"] / max(1, l["cg_iters"]))
     print(json.dumps(report))
-    # the true residual decides: the learned path never converges to
-    # a different answer, and it must not be pathologically slower
-    assert lt < 2.0 * jt
+    assert not l["fallback"]
+    # CG work per outer iteration (the outer trajectories of the three legs
+    # differ within the stopping rule's tolerance, so totals mix in how many
+    # outer iterations each took): the learned diagonal beats Jacobi-CG and
+    # does not lose to the best method without it
+    per = {k: r["cg_iters"] / r["oiter"] for k, r in res.items()}
+    best_per = min(per["NONE"], per["JACOBI"])
+    assert per["JACOBI"] >= 1.15 * per["LEARNED"]
+    assert per["LEARNED"] <= 1.02 * best_per

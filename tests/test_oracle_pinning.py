@@ -206,6 +206,24 @@ def test_two_holes_dqg_is_exact():
     assert abs(st["energy_primal"] - E) < 1e-4
 
 
+def test_resumed_loop_is_the_same_iteration():
+    """A solve split into two calls (the second resuming mu, the iteration
+    count and the errors of the first) walks the same trajectory as one call:
+    same outer iteration count (to one iteration: the OpenMP reductions do
+    not fix their summation order) and energy."""
+    L = orc.Layout([4], 2, 2, None, "DQG")
+    h, eri = _ham(L, 0)
+    c, b = L.build_c(h, eri), L.build_b()
+    x, z, y = L.guess_random()
+    one = L.bpsdp(c, b, x.copy(), z.copy(), y.copy(), maxiter=20000)
+    first = L.bpsdp(c, b, x, z, y, maxiter=150)
+    assert first["oiter"] == 150 and not first["converged"]
+    second = L.bpsdp(c, b, x, z, y, maxiter=20000, resume=first)
+    assert one["converged"] and second["converged"]
+    assert abs(second["oiter"] - one["oiter"]) <= 2
+    assert abs(second["energy_primal"] - one["energy_primal"]) < 1e-8
+
+
 def test_mlp_gradient_matches_finite_differences():
     rng = np.random.default_rng(4)
     n, nin, nh = 13, 6, 8

@@ -31,7 +31,8 @@ class Params(ctypes.Structure):
                 ("train_lr", ctypes.c_double), ("eig_warm_sweeps", ctypes.c_int),
                 ("eig_inner_sweeps", ctypes.c_int), ("cg_extrapolate", ctypes.c_int),
                 ("mu0", ctypes.c_double), ("oiter0", ctypes.c_int), ("ep0", ctypes.c_double),
-                ("ed0", ctypes.c_double)]
+                ("ed0", ctypes.c_double), ("mlp_base", ctypes.c_int),
+                ("eig_pair_order", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -76,6 +77,9 @@ SIGNATURES = {
     "v2g_build_b": (ctypes.c_int, [_VP, _DP]),
     "v2g_build_c": (ctypes.c_int, [_VP, _DP, _DP, _DP]),
     "v2g_guess_random": (ctypes.c_int, [_VP, _DP, _DP, _DP]),
+    "v2g_build_b_device": (ctypes.c_int, [_VP, _DP]),
+    "v2g_build_c_device": (ctypes.c_int, [_VP, _DP, _DP, _DP]),
+    "v2g_guess_random_device": (ctypes.c_int, [_VP, _DP, _DP, _DP]),
     "v2g_mlp_num_params": (ctypes.c_int, []),
     "v2g_mlp_init": (ctypes.c_int, [_VP, ctypes.c_ulonglong]),
     "v2g_mlp_get_weights": (ctypes.c_int, [_VP, _DP]),
@@ -83,7 +87,12 @@ SIGNATURES = {
     "v2g_mlp_loss_grad": (ctypes.c_int, [_VP, ctypes.c_int, _DP, _DP, _DP, _DP,
                                          ctypes.POINTER(ctypes.c_double)]),
     "v2g_mlp_adam_step": (ctypes.c_int, [_VP, _DP, ctypes.c_double]),
+    "v2g_mlp_spectral_grad": (ctypes.c_int, [_VP, _DP, _DP, _DP, _DP, _DP,
+                                             ctypes.POINTER(ctypes.c_double)]),
     "v2g_mlp_minv": (ctypes.c_int, [_VP, _DP, _DP, _DP]),
+    "v2g_get_opdm": (ctypes.c_int, [_VP, _DP, _DP]),
+    "v2g_get_tpdm": (ctypes.c_int, [_VP, _DP, _DP]),
+    "v2g_write_fcidump_rdm": (ctypes.c_int, [_VP, _DP, ctypes.c_char_p]),
     "v2g_last_error": (ctypes.c_char_p, []),
     "v2g_version": (ctypes.c_char_p, []),
 }

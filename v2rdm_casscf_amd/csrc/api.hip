@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -16,6 +17,7 @@
 #include "../../include/v2g.h"
 #include "devlayout.hpp"
 #include "eig.hpp"
+#include "inputs.hpp"
 #include "kernels.hpp"
 #include "layout.hpp"
 #include "mlp.hpp"
@@ -76,6 +78,7 @@ struct v2g_solver {
     DevBuf<v2g::DevLayout> d_lay;
     DevBuf<v2g::Segment> d_rowsegs, d_xsegs, d_t2segs;
     DevBuf<int64_t> d_rowtiles, d_xtiles, d_t2tiles;
+    DevBuf<int2> d_g2idx;
     v2g::MapPlan plan;
     // CG
     DevBuf<double> r, p, z, Ap, t, partial, scal, jacobi, user_minv, rowpart;
@@ -87,6 +90,12 @@ struct v2g_solver {
     int mlp_outer_trained = 0;  // outer iterations of MLP training done so far
     bool mlp_fallback = false, mlp_first_learned = true;
     int mlp_last_base_iters = 0;   // CG iterations of the last training solve
+    int mlp_jacobi_base = 0;       // 1: learned scaling of the Jacobi diagonal
+    // A/B guard of the learned diagonal: CG convergence rates ln(r0 / r) per
+    // iteration of the last base solves and of the first learned solves
+    double base_rate_sum = 0.0, learned_rate_sum = 0.0;
+    int base_rate_n = 0, learned_rate_n = 0;
+    double last_cg_nrm0 = 0.0;     // ||r|| at the start of the last CG solve
     // eigen split
     std::vector<v2g::EigBlock> small, large;
     DevBuf<v2g::EigBlock> d_small, d_large;
@@ -136,7 +145,7 @@ struct v2g_solver {
     bool have_rownorm = false;
     DevBuf<unsigned char> code;
     DevBuf<float> W, mlp_part, mlp_lpart, feat, tgt, wt;
-    DevBuf<double> am, av, grad, dloss, scales, yprev, r0, dy, lminv, vtmp, kd;
+    DevBuf<double> am, av, grad, dloss, scales, yprev, lminv, vtmp, kd, kd2, rhi, ones;
     DevBuf<int64_t> bidx;
     int adam_t = 0, batch_cap = 0;
     bool mlp_ready = false;
@@ -172,7 +181,7 @@ struct v2g_solver {
     void mlp_init(unsigned long long seed);
     void mlp_scales(const double *rhs, const double *y);
     void mlp_infer(const double *rhs, const double *y);
-    double mlp_train(const double *rhs, const double *rsample, int nb, int steps, double lr,
+    double mlp_train(const double *rhs, const double *rlo, int nb, int steps, double lr,
                      std::mt19937_64 &rng);
     // the device vectors and scalars one CG solve works in
     struct CgBufs {
@@ -180,11 +189,11 @@ struct v2g_solver {
     };
     CgBufs own_cg_bufs() { return {r.p, p.p, Ap.p, t.p, partial.p, scal.p, rowpart.p}; }
     int cg(const double *rhs, double *y, double tol, int maxiter, int precond, int flexible,
-           double *res, double *r0_out = nullptr);
+           double *res);
     int cg_with(const CgBufs &B, bool graphs, const double *rhs, double *y, double tol, int maxiter,
-                const double *minv, int flexible, double *res, double *r0_out);
+                const double *minv, int flexible, double *res);
     void update_xz(double mu, const double *y, const double *c, double *x, double *z,
-                   int warm_sweep_cap = 0, int inner_sweep_cap = 0);
+                   int warm_sweep_cap = 0, int inner_sweep_cap = 0, int cross_order = 0);
     double dot(const double *a, const double *b, int64_t n);
 };
 
@@ -261,7 +270,15 @@ v2g_solver::v2g_solver(const v2g_layout_desc &d)
     std::vector<int64_t> ts;
     plan.dlay = d_lay.p;
     plan.row_tiles = tiles(L->rowsegs, ts);
-    d_rowsegs.upload(L->rowsegs);
+    // G2 row segments get a slice of the precomputed column table (aux)
+    std::vector<v2g::Segment> rsegs = L->rowsegs;
+    int64_t g2rows = 0;
+    for (auto &s : rsegs) {
+        const bool g2 = s.kind == v2g::RF_G2AB || s.kind == v2g::RF_G2AA;
+        s.aux = g2 && !s.wave ? g2rows : -1;
+        if (s.aux >= 0) g2rows += s.count;
+    }
+    d_rowsegs.upload(rsegs);
     d_rowtiles.upload(ts);
     plan.rowsegs = d_rowsegs.p;
     plan.row_tile_start = d_rowtiles.p;
@@ -278,6 +295,30 @@ v2g_solver::v2g_solver(const v2g_layout_desc &d)
     plan.t2segs = d_t2segs.p;
     plan.t2_tile_start = d_t2tiles.p;
     plan.nt2segs = (int)L->xsegs_t2.size();
+    // the D1a / D1b blocks must form one range of x that no other block
+    // enters, and every column must fit 30 bits, for the packed G2 table
+    int64_t d1lo = INT64_MAX, d1hi = -1;
+    for (int hh = 0; hh < L->nirrep; hh++) {
+        const int64_t nn = (int64_t)o.amopi[hh] * o.amopi[hh];
+        if (!nn) continue;
+        d1lo = std::min(d1lo, L->xoff[v2g::XB_D1A][hh]);
+        d1hi = std::max(d1hi, L->xoff[v2g::XB_D1B][hh] + nn);
+    }
+    bool packable = g2rows > 0 && d1hi > d1lo && L->dimx < ((int64_t)1 << 30);
+    for (size_t bi = 0; packable && bi < L->block_dims.size(); bi++) {
+        const int64_t lo = L->block_offs[bi], hi = lo + (int64_t)L->block_dims[bi] * L->block_dims[bi];
+        bool d1 = false;
+        for (int hh = 0; hh < L->nirrep; hh++)
+            d1 = d1 || lo == L->xoff[v2g::XB_D1A][hh] || lo == L->xoff[v2g::XB_D1B][hh];
+        if (!d1 && hi > lo && lo < d1hi && hi > d1lo) packable = false;
+    }
+    if (packable) {
+        d_g2idx.alloc(g2rows);
+        plan.g2idx = d_g2idx.p;
+        v2g::build_g2_index(plan, rsegs, d1lo, d1hi, d_g2idx.p, st);
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(st));
+    }
 
     const int64_t ncon = L->ncon, dimx = L->dimx;
     r.alloc(ncon);
@@ -436,12 +477,13 @@ void v2g_solver::ensure_mlp()
     av.alloc(v2g::kMlpParams);
     grad.alloc(v2g::kMlpParams);
     dloss.alloc(1);
-    scales.alloc(4);
+    scales.alloc(8);
     yprev.alloc(n);
-    r0.alloc(n);
-    dy.alloc(n);
     lminv.alloc(n);
     vtmp.alloc(n);
+    kd.alloc(n);
+    kd2.alloc(n);
+    rhi.alloc(n);
     mlp_ready = true;
     mlp_init(12345);
 }
@@ -455,7 +497,8 @@ void v2g_solver::mlp_init(unsigned long long seed)
     for (int i = 0; i < in * hd; i++) w[i] = (float)(nd(rng) / std::sqrt((double)in));
     const int oW2 = in * hd + hd;
     for (int i = 0; i < hd * hd; i++) w[oW2 + i] = (float)(nd(rng) / std::sqrt((double)hd));
-    // output layer (w3, b3) stays zero: g = 0, the preconditioner starts as Jacobi
+    // output layer (w3, b3) stays zero: g = 0, the preconditioner starts as its
+    // base (the identity, i.e. the reference's plain CG, or Jacobi)
     HIPCHK(hipMemcpyAsync(W.p, w.data(), sizeof(float) * w.size(), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemsetAsync(am.p, 0, sizeof(double) * v2g::kMlpParams, st));
     HIPCHK(hipMemsetAsync(av.p, 0, sizeof(double) * v2g::kMlpParams, st));
@@ -465,6 +508,8 @@ void v2g_solver::mlp_init(unsigned long long seed)
     mlp_fallback = false;
     mlp_first_learned = true;
     mlp_last_base_iters = 0;
+    base_rate_sum = learned_rate_sum = 0.0;
+    base_rate_n = learned_rate_n = 0;
 }
 
 // scales[0] = 1/rms(rhs), scales[1] = 1/rms(y)
@@ -479,57 +524,79 @@ void v2g_solver::mlp_scales(const double *rhs, const double *y)
 void v2g_solver::mlp_infer(const double *rhs, const double *y)
 {
     mlp_scales(rhs, y);
-    v2g::mlp_minv(L->ncon, code.p, rownorm.p, rhs, y, W.p, scales.p, lminv.p, 3.0, st);
+    v2g::mlp_minv(L->ncon, code.p, rownorm.p, rhs, y, W.p, scales.p, lminv.p, 3.0, mlp_jacobi_base,
+                  st);
 }
 
-// training on residual samples of the Jacobi-CG solves: maximize the
-// step-quality J(s) = (r.Sr)^2 / ((Sr).A A^T (Sr)) of the learned diagonal
-// S = diag(s) (scale-invariant in s; Jacobi is the starting point).  Each
-// step: inference on every row, one A A^T product, two dots, a sampled
-// batch with the exact upstream gradient of -log J, backprop and Adam.
-// Returns log(J_learned / J_jacobi) on the last sample.
-double v2g_solver::mlp_train(const double *rhs, const double *rsample, int nb, int steps,
-                             double lr, std::mt19937_64 &rng)
+// Online training of the learned diagonal S on the spectral objective
+// L = log rho(r_hi) - log rho(r_lo), rho(r) = (Sr).K(Sr) / (r.Sr), K = A A^T:
+// r_lo is the final residual of the base CG solve just done (its slowest
+// modes), r_hi comes from power iterations r <- K S r (the fastest).  L
+// estimates log cond(S K); each Adam step lowers it.  Every step: inference
+// on all rows, two K products, four dots, a sampled batch with the exact
+// upstream gradient, backprop and Adam.  Returns L after the last step minus
+// L of the base diagonal on the same probes (negative: the learned diagonal
+// is better conditioned on them).
+double v2g_solver::mlp_train(const double *rhs, const double *rlo, int nb, int steps, double lr,
+                             std::mt19937_64 &rng)
 {
     const int64_t n = L->ncon;
     if (nb > batch_cap) {
         bidx.alloc(nb);
         feat.alloc((size_t)nb * v2g::kMlpIn);
-        tgt.alloc(nb);
         wt.alloc(nb);
         mlp_part.alloc((size_t)v2g::mlp_grad_blocks(nb) * v2g::kMlpParams);
         mlp_lpart.alloc(v2g::mlp_grad_blocks(nb));
         batch_cap = nb;
     }
-    if (!kd.p) kd.alloc(n);
-    auto jval = [&](const double *sv, double &a, double &b) {
-        v2g::vec_mul(n, sv, rsample, vtmp.p, st);
+    auto kprod = [&](const double *sv, const double *r, double *out) {   // out = K (s o r)
+        v2g::vec_mul(n, sv, r, vtmp.p, st);
         v2g::launch_atu(plan, vtmp.p, t.p, st);
-        v2g::launch_au(plan, t.p, kd.p, st);
-        a = dot(rsample, vtmp.p, n);
-        b = dot(vtmp.p, kd.p, n);
-        return (b > 0.0) ? a * a / b : 0.0;
+        v2g::launch_au(plan, t.p, out, st);
     };
-    double aj, bj;
-    const double jjac = jval(jacobi.p, aj, bj);
+    auto rho_terms = [&](const double *sv, const double *r, double *kdv, double &a, double &bq) {
+        kprod(sv, r, kdv);
+        a = dot(r, vtmp.p, n);       // r . S r
+        bq = dot(vtmp.p, kdv, n);    // (S r) . K (S r)
+    };
+    auto objective = [&](const double *sv, double &ah, double &bh, double &al, double &bl) {
+        rho_terms(sv, rhi.p, kd.p, ah, bh);
+        rho_terms(sv, rlo, kd2.p, al, bl);
+        return (ah > 0 && bh > 0 && al > 0 && bl > 0) ? std::log(bh / ah) - std::log(bl / al) : 0.0;
+    };
+    // r_hi: power iterations of K S from the right-hand side
+    mlp_infer(rhs, yprev.p);
+    HIPCHK(hipMemcpyAsync(rhi.p, rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    for (int it = 0; it < 6; it++) {
+        kprod(lminv.p, rhi.p, kd.p);
+        const double nrm = std::sqrt(dot(kd.p, kd.p, n));
+        if (!(nrm > 0.0)) break;
+        v2g::vec_axpby(n, 1.0 / nrm, kd.p, 0.0, nullptr, rhi.p, st);
+    }
+    // the base diagonal (g = 0) on the same probes, for the reported change
+    if (!mlp_jacobi_base && !ones.p) {
+        ones.alloc(n);
+        v2g::vec_fill(n, 1.0, ones.p, st);
+    }
+    double ah, bh, al, bl;
+    const double l0 = objective(mlp_jacobi_base ? jacobi.p : ones.p, ah, bh, al, bl);
     std::uniform_int_distribution<int64_t> ud(0, n - 1);
     std::vector<int64_t> idx(nb);
-    double jl = jjac;
+    double l = l0;
     for (int k = 0; k < steps; k++) {
         mlp_infer(rhs, yprev.p);              // s = lminv, scales[0..1]
-        double a, b;
-        jl = jval(lminv.p, a, b);
-        double ab[2] = {a, b};
+        l = objective(lminv.p, ah, bh, al, bl);
+        double ab[4] = {ah, bh, al, bl};
         HIPCHK(hipMemcpyAsync(scales.p + 2, ab, sizeof(ab), hipMemcpyHostToDevice, st));
         for (auto &i : idx) i = ud(rng);
         HIPCHK(hipMemcpyAsync(bidx.p, idx.data(), sizeof(int64_t) * nb, hipMemcpyHostToDevice, st));
-        v2g::mlp_batch_j(nb, bidx.p, code.p, rownorm.p, rhs, yprev.p, rsample, lminv.p, kd.p,
-                         scales.p, feat.p, wt.p, 3.0, (double)n / nb, st);
+        v2g::mlp_batch_spec(nb, bidx.p, code.p, rownorm.p, rhs, yprev.p, rhi.p, kd.p, rlo, kd2.p,
+                            lminv.p, scales.p, feat.p, wt.p, 3.0, (double)n / nb, mlp_jacobi_base, st);
         v2g::mlp_grad(nb, feat.p, nullptr, wt.p, W.p, mlp_part.p, mlp_lpart.p, grad.p, dloss.p, st);
         v2g::mlp_adam(W.p, grad.p, am.p, av.p, lr, ++adam_t, st);
     }
     HIPCHK(hipStreamSynchronize(st));
-    return (jjac > 0.0 && jl > 0.0) ? std::log(jl / jjac) : 0.0;
+    return l - l0;
 }
 
 double v2g_solver::dot(const double *a, const double *b, int64_t n)
@@ -543,7 +610,7 @@ double v2g_solver::dot(const double *a, const double *b, int64_t n)
 }
 
 int v2g_solver::cg(const double *rhs, double *y, double tolv, int maxiter, int precond,
-                   int flexible, double *res, double *r0_out)
+                   int flexible, double *res)
 {
     const double *minv = nullptr;
     if (precond == 1) { ensure_jacobi(); minv = jacobi.p; }
@@ -553,19 +620,17 @@ int v2g_solver::cg(const double *rhs, double *y, double tolv, int maxiter, int p
     } else if (precond == 4) {
         minv = lminv.p;  // the learned diagonal from mlp_infer
     }
-    return cg_with(own_cg_bufs(), use_graphs, rhs, y, tolv, maxiter, minv, flexible, res, r0_out);
+    return cg_with(own_cg_bufs(), use_graphs, rhs, y, tolv, maxiter, minv, flexible, res);
 }
 
 int v2g_solver::cg_with(const CgBufs &B, bool graphs, const double *rhs, double *y, double tolv,
-                        int maxiter, const double *minv, int flexible, double *res, double *r0_out)
+                        int maxiter, const double *minv, int flexible, double *res)
 {
     const int64_t n = L->ncon;
     // Ap = A A^T y ; r = b - Ap ; p = M^-1 r   (cg_solver.cc:141-150)
     v2g::launch_atu(plan, y, B.t, st);
     v2g::launch_au(plan, B.t, B.Ap, st);
     v2g::cg_init(n, rhs, B.Ap, B.r, B.p, minv, nullptr, B.partial, B.scal, tolv, maxiter, st);
-    if (r0_out)
-        HIPCHK(hipMemcpyAsync(r0_out, B.r, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     // one chunk of kCgChunk (even) iterations; every kernel argument is fixed,
     // the iteration number and the done flags live in scal
     auto enqueue_chunk = [&]() {
@@ -617,6 +682,7 @@ int v2g_solver::cg_with(const CgBufs &B, bool graphs, const double *rhs, double 
         if (it > maxiter + 2 * kCgChunk) break;  // defensive: cannot happen
     }
     if (res) *res = hs[v2g::CG_NRM];
+    last_cg_nrm0 = hs[v2g::CG_NRM0];
     return (int)hs[v2g::CG_ITER];
 }
 
@@ -625,7 +691,7 @@ int v2g_solver::cg_with(const CgBufs &B, bool graphs, const double *rhs, double 
 // becomes exact at the fixed point while early calls stay cheap.  x is always
 // rebuilt PSD from the current V; z = mu x - M keeps M exact.
 void v2g_solver::update_xz(double mu, const double *y, const double *c, double *x, double *zv,
-                           int warm_sweep_cap, int inner_sweep_cap)
+                           int warm_sweep_cap, int inner_sweep_cap, int cross_order)
 {
     v2g::launch_atu(plan, y, ATy.p, st);
     v2g::eig_small(d_small.p, (int)small.size(), ATy.p, c, x, zv, mu, st);
@@ -640,7 +706,10 @@ void v2g_solver::update_xz(double mu, const double *y, const double *c, double *
     }
     const int nl = (int)large.size();
     v2g::eig_build_m(d_large.p, nl, max_N, ATy.p, c, x, mu, Mw.p, Aw.p, Vw.p, cold, st);
-    v2g::eig_tol(d_large.p, nl, Mw.p, tol.p, frop.p, st);
+    // a split that may stop after capped sweeps (in-loop tracking) rotates
+    // above 2e-14 ||M||_F; a split that must converge uses the tight threshold
+    const bool capped = !cold && (warm_sweep_cap > 0 || inner_sweep_cap > 0);
+    v2g::eig_tol(d_large.p, nl, Mw.p, tol.p, frop.p, capped ? 0 : 1, st);
     if (!cold) {
         // A = V^T (M V)
         if (!d_gemm_warm1.p) {
@@ -674,8 +743,11 @@ void v2g_solver::update_xz(double mu, const double *y, const double *c, double *
                 if (rnd >= lane_nb[l] - 1) continue;
                 const v2g::EigBlock *bl = d_large.p + lane_off[l];
                 hipStream_t s = l ? lane_st[l] : st;
+                // a capped tracking sweep may take the cross-pair order after
+                // its first round (every pair is still visited once)
+                const int bip = (capped && cross_order && rnd > 0) ? 1 : 0;
                 v2g::eig_pairs(bl, lane_n[l], lane_pairs[l], rnd, Aw.p, Qbuf.p, active.p,
-                               tol.p + lane_off[l], counter.p, inner, pair_wide ? 1 : 0, s);
+                               tol.p + lane_off[l], counter.p, inner, pair_wide ? 1 : 0, bip, s);
                 if (rnd == 0 && l % 2 == 0 && l + 1 < nlanes) {
                     // lane l + 1 starts once lane l's first pair problems are
                     // done, so in steady state each of the two runs its tile
@@ -741,6 +813,21 @@ void v2g_solver::update_xz(double mu, const double *y, const double *c, double *
     catch (const std::invalid_argument &e) { g_err = e.what(); return -1; }              \
     catch (const HipError &e) { g_err = e.what(); return -2; }                           \
     catch (const std::exception &e) { g_err = e.what(); return -3; }
+
+namespace {
+
+// host copy of the primal block `b` of irrep `hh` (n x n entries)
+std::vector<double> block_to_host(v2g_handle h, const double *x, int b, int hh, int64_t nn)
+{
+    std::vector<double> v((size_t)nn);
+    const int64_t o = h->L->xoff[b][hh];
+    if (nn == 0) return v;
+    if (o < 0) throw std::invalid_argument("primal block absent from this layout");
+    HIPCHK(hipMemcpy(v.data(), x + o, sizeof(double) * nn, hipMemcpyDeviceToHost));
+    return v;
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -935,13 +1022,18 @@ int v2g_bpsdp(v2g_handle h, const v2g_params *prm, const double *c, const double
     v2g_stats s{};
     bool stop = false;
     const bool learned = prm->precond == 3;
-    if (learned) h->ensure_mlp();
+    if (learned) {
+        h->ensure_mlp();
+        h->mlp_jacobi_base = prm->mlp_base == 1 ? 1 : 0;
+    }
+    // the learned diagonal's base method: plain CG (identity) or Jacobi-CG
+    const int base_pc = (learned && h->mlp_jacobi_base) ? 1 : 0;
     std::mt19937_64 rng(20240607ULL);
     long long cg_train = 0;
     int n_train = 0;
     // the A/B guard state lives in the handle, so a later call keeps the verdict
     bool &fallback = h->mlp_fallback, &first_learned = h->mlp_first_learned;
-    int &last_jacobi_iters = h->mlp_last_base_iters;
+    int &last_base_iters = h->mlp_last_base_iters;
     double last_loss = 0.0;
     do {
         const double t1 = now();
@@ -962,7 +1054,7 @@ int v2g_bpsdp(v2g_handle h, const v2g_params *prm, const double *c, const double
         const bool training = learned && h->mlp_outer_trained < prm->train_iters;
         if (learned) {
             if (training || fallback) {
-                pc = 1;
+                pc = base_pc;
                 flex = 0;
             } else {
                 h->mlp_infer(h->Bv.p, y);
@@ -983,34 +1075,48 @@ int v2g_bpsdp(v2g_handle h, const v2g_params *prm, const double *c, const double
         }
         if (training)
             HIPCHK(hipMemcpyAsync(h->yprev.p, y, sizeof(double) * ncon, hipMemcpyDeviceToDevice, st));
-        const int iiter = h->cg(h->Bv.p, y, cgtol, prm->cg_maxiter, pc, flex, &res,
-                                training ? h->r0.p : nullptr);
+        const int iiter = h->cg(h->Bv.p, y, cgtol, prm->cg_maxiter, pc, flex, &res);
         cgtot += iiter;
+        // convergence rate of this solve: ln(||r_0|| / ||r||) per iteration
+        const double rate = (iiter > 0 && res > 0.0 && h->last_cg_nrm0 > res)
+                                ? std::log(h->last_cg_nrm0 / res) / iiter : -1.0;
         if (training) {
-            // residual samples: the solve's initial residual and its final one
-            // (the slowest modes of the Jacobi-preconditioned operator)
-            const double *rs = (oiter & 1) ? h->r.p : h->r0.p;
-            last_loss = h->mlp_train(h->Bv.p, rs, prm->train_batch > 0 ? prm->train_batch : 16384,
+            // the base solve's final residual carries its slowest modes
+            last_loss = h->mlp_train(h->Bv.p, h->r.p, prm->train_batch > 0 ? prm->train_batch : 16384,
                                      prm->train_steps > 0 ? prm->train_steps : 10,
                                      prm->train_lr > 0 ? prm->train_lr : 2e-3, rng);
             cg_train += iiter;
             n_train++;
             h->mlp_outer_trained++;
-            last_jacobi_iters = iiter;
+            last_base_iters = iiter;
+            // the base method's rate over the second half of the training phase
+            if (rate > 0.0 && 2 * h->mlp_outer_trained > prm->train_iters) {
+                h->base_rate_sum += rate;
+                h->base_rate_n++;
+            }
         } else if (pc == 4) {
-            // the true residual decides: abandon the learned diagonal if it
-            // failed to converge, or if its first solve needed clearly more
-            // iterations than the last Jacobi solve
+            // the true residual decides: abandon the learned diagonal if a
+            // solve fails to converge, or if over its first kGuard solves it
+            // converges clearly more slowly per iteration than its base did
+            constexpr int kGuard = 4;
             if (iiter >= prm->cg_maxiter || !std::isfinite(res)) fallback = true;
-            if (first_learned && iiter > 1.15 * last_jacobi_iters + 2) fallback = true;
-            first_learned = false;
+            if (first_learned && rate > 0.0) {
+                h->learned_rate_sum += rate;
+                if (++h->learned_rate_n == kGuard) {
+                    first_learned = false;
+                    if (h->base_rate_n > 0 && h->learned_rate_sum / kGuard <
+                                                  0.9 * h->base_rate_sum / h->base_rate_n)
+                        fallback = true;
+                }
+            }
         }
         s.precond_active = pc == 4 ? 3 : pc;
         const double t2 = now();
         tcg += t2 - t1;
         if (capped_loop)
             HIPCHK(hipMemcpyAsync(h->xprev.p, x, sizeof(double) * dimx, hipMemcpyDeviceToDevice, st));
-        h->update_xz(mu, y, c, x, z, prm->eig_warm_sweeps, prm->eig_inner_sweeps);
+        h->update_xz(mu, y, c, x, z, prm->eig_warm_sweeps, prm->eig_inner_sweeps,
+                     prm->eig_pair_order);
         HIPCHK(hipStreamSynchronize(st));
         txz += now() - t2;
         // ed = ||A^T y - c + z||, ep = ||A x - b||  (v2rdm_solver.cc:1829-1837)
@@ -1153,6 +1259,50 @@ int v2g_build_c(v2g_handle h, const double *h1, const double *eri, double *c)
     API_END
 }
 
+int v2g_build_b_device(v2g_handle h, double *b)
+{
+    API_BEGIN
+    if (!h || !b) throw std::invalid_argument("null argument");
+    const v2g::Layout &L = *h->L;
+    const double ms = (L.opt.multiplicity - 1.0) / 2.0;
+    v2g::build_b_device(h->d_lay.p, h->d_rowsegs.p, (int)L.rowsegs.size(), L.ncon, ms, b, h->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->st));
+    API_END
+}
+
+int v2g_build_c_device(v2g_handle h, const double *h1, const double *eri, double *c)
+{
+    API_BEGIN
+    if (!h || !h1 || !eri || !c) throw std::invalid_argument("null argument");
+    const v2g::Layout &L = *h->L;
+    int mgab = 0, mgaa = 0, mm = 0;
+    for (int hh = 0; hh < L.nirrep; hh++) {
+        mgab = std::max(mgab, L.gems_ab[hh]);
+        mgaa = std::max(mgaa, L.gems_aa[hh]);
+        mm = std::max(mm, L.opt.amopi[hh]);
+    }
+    v2g::build_c_device(h->d_lay.p, L.nirrep, mgab, mgaa, mm, L.dimx, h1, eri, c, h->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->st));
+    API_END
+}
+
+int v2g_guess_random_device(v2g_handle h, double *x, double *z, double *y)
+{
+    API_BEGIN
+    if (!h || !x || !z || !y) throw std::invalid_argument("null argument");
+    const v2g::Layout &L = *h->L;
+    const int64_t total = 2 * L.dimx + L.ncon;
+    const std::vector<uint32_t> starts = v2g::guess_chunk_starts(total);
+    DevBuf<uint32_t> d_starts;
+    d_starts.upload(starts);
+    v2g::guess_device(d_starts.p, total, L.dimx, x, z, y, h->st);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->st));
+    API_END
+}
+
 int v2g_guess_random(v2g_handle h, double *x, double *z, double *y)
 {
     API_BEGIN
@@ -1164,6 +1314,126 @@ int v2g_guess_random(v2g_handle h, double *x, double *z, double *y)
         z[i] = ((double)rand() / RAND_MAX - 1.0) * 2.0;
     }
     for (int64_t i = 0; i < L.ncon; i++) y[i] = ((double)rand() / RAND_MAX - 1.0) * 2.0;
+    API_END
+}
+
+// RDM outputs (host side; the RDMs are small next to x)
+
+int v2g_get_opdm(v2g_handle h, const double *x, double *opdm)
+{
+    API_BEGIN
+    if (!h || !x || !opdm) throw std::invalid_argument("null argument");
+    HIPCHK(hipStreamSynchronize(h->st));
+    const v2g::Layout &L = *h->L;
+    int64_t o = 0;
+    for (int hh = 0; hh < L.nirrep; hh++) {
+        const int64_t n = L.opt.amopi[hh], nn = n * n;
+        const auto a = block_to_host(h, x, v2g::XB_D1A, hh, nn);
+        const auto b = block_to_host(h, x, v2g::XB_D1B, hh, nn);
+        for (int64_t i = 0; i < nn; i++) opdm[o + i] = a[i] + b[i];
+        o += nn;
+    }
+    API_END
+}
+
+int v2g_get_tpdm(v2g_handle h, const double *x, double *tpdm)
+{
+    API_BEGIN
+    if (!h || !x || !tpdm) throw std::invalid_argument("null argument");
+    const v2g::Layout &L = *h->L;
+    if (L.nirrep > 1)   // python_api.cc:400-402
+        throw std::invalid_argument("not sure how CIWavefunction stores TPDM with symmetry");
+    HIPCHK(hipStreamSynchronize(h->st));
+    const int64_t a = L.amo, gab = L.gems_ab[0], gaa = L.gems_aa[0];
+    const auto ab = block_to_host(h, x, v2g::XB_D2AB, 0, gab * gab);
+    const auto aa = block_to_host(h, x, v2g::XB_D2AA, 0, gaa * gaa);
+    const auto bb = block_to_host(h, x, v2g::XB_D2BB, 0, gaa * gaa);
+    auto GAB = [&](int64_t p, int64_t q) { return (int64_t)L.gab[p * a + q]; };
+    auto GAA = [&](int64_t p, int64_t q) { return (int64_t)L.gaa[p * a + q]; };
+    // spin-summed <a+_i a+_j a_l a_k> from the ab block (both orderings) and,
+    // for i != j, k != l, the aa and bb blocks with their pair signs
+    auto d2 = [&](int64_t i, int64_t j, int64_t k, int64_t l) {
+        double v = ab[GAB(i, j) * gab + GAB(k, l)] + ab[GAB(j, i) * gab + GAB(l, k)];
+        if (i != j && k != l) {
+            const double sg = ((i > j) ? -1.0 : 1.0) * ((k > l) ? -1.0 : 1.0);
+            const int64_t e = GAA(i, j) * gaa + GAA(k, l);
+            v += sg * (aa[e] + bb[e]);
+        }
+        return v;
+    };
+    for (int64_t i = 0; i < a; i++)
+        for (int64_t j = 0; j < a; j++)
+            for (int64_t k = 0; k < a; k++)
+                for (int64_t l = 0; l < a; l++) {
+                    // python_api.cc:412-452: tpdm[ik][jl] = (D(ij,kl) + D(il,kj)) / 2,
+                    // the second term with its D2ab pair in (il,kj) / (li,jk) order
+                    double ikjl = d2(i, j, k, l);
+                    double iklj = ab[GAB(i, l) * gab + GAB(k, j)] + ab[GAB(l, i) * gab + GAB(j, k)];
+                    if (i != l && k != j) {
+                        const double sg = ((i > l) ? -1.0 : 1.0) * ((k > j) ? -1.0 : 1.0);
+                        const int64_t e = GAA(i, l) * gaa + GAA(k, j);
+                        ikjl += sg * (aa[e] + bb[e]);
+                    }
+                    tpdm[(i * a + k) * (a * a) + j * a + l] = 0.5 * (ikjl + iklj);
+                }
+    API_END
+}
+
+int v2g_write_fcidump_rdm(v2g_handle h, const double *x, const char *path)
+{
+    API_BEGIN
+    if (!h || !x || !path) throw std::invalid_argument("null argument");
+    HIPCHK(hipStreamSynchronize(h->st));
+    const v2g::Layout &L = *h->L;
+    const int64_t a = L.amo;
+    FILE *f = std::fopen(path, "wb");
+    if (!f) throw std::invalid_argument(std::string("cannot open ") + path);
+    std::unique_ptr<FILE, int (*)(FILE *)> guard(f, std::fclose);
+    auto rec = [&](double v, int p, int q, int r, int s) {
+        const int idx[4] = {p, q, r, s};
+        if (std::fwrite(&v, sizeof(double), 1, f) != 1 || std::fwrite(idx, sizeof(int), 4, f) != 4)
+            throw std::runtime_error("short write");
+    };
+    // two-electron part (fcidump.cc:183-230): per irrep, every ab geminal pair;
+    // active orbitals are numbered from 1 in Pitzer (irrep-major) order
+    for (int hh = 0; hh < L.nirrep; hh++) {
+        const int64_t g = L.gems_ab[hh], ga = L.gems_aa[hh];
+        const auto ab = block_to_host(h, x, v2g::XB_D2AB, hh, g * g);
+        const auto aa = block_to_host(h, x, v2g::XB_D2AA, hh, ga * ga);
+        const auto bb = block_to_host(h, x, v2g::XB_D2BB, hh, ga * ga);
+        const int *bp = L.bab.data() + L.bab_off[hh];
+        for (int64_t ij = 0; ij < g; ij++) {
+            const int i = bp[ij] & 0xffff, j = bp[ij] >> 16;
+            const int64_t ji = L.gab[(int64_t)j * a + i];
+            for (int64_t kl = 0; kl < g; kl++) {
+                const int k = bp[kl] & 0xffff, l = bp[kl] >> 16;
+                const int64_t lk = L.gab[(int64_t)l * a + k];
+                double v = ab[ij * g + kl] + ab[ji * g + lk];
+                if (i != j && k != l) {
+                    const int64_t e = (int64_t)L.gaa[(int64_t)i * a + j] * ga + L.gaa[(int64_t)k * a + l];
+                    const double sg = ((i > j) ? -1.0 : 1.0) * ((k > l) ? -1.0 : 1.0);
+                    v += sg * (aa[e] + bb[e]);
+                }
+                rec(v, i + 1, j + 1, k + 1, l + 1);
+            }
+        }
+    }
+    // one-electron part (fcidump.cc:238-268): totally symmetric ab geminals
+    const int64_t g0 = L.gems_ab[0];
+    const int *bp0 = L.bab.data() + L.bab_off[0];
+    std::vector<std::vector<double>> d1a(L.nirrep), d1b(L.nirrep);
+    for (int hh = 0; hh < L.nirrep; hh++) {
+        const int64_t n = L.opt.amopi[hh];
+        d1a[hh] = block_to_host(h, x, v2g::XB_D1A, hh, n * n);
+        d1b[hh] = block_to_host(h, x, v2g::XB_D1B, hh, n * n);
+    }
+    for (int64_t ij = 0; ij < g0; ij++) {
+        const int i = bp0[ij] & 0xffff, j = bp0[ij] >> 16;
+        const int hi = L.sym[i];
+        if (hi != L.sym[j]) continue;
+        const int64_t n = L.opt.amopi[hi], ii = i - L.poff[hi], jj = j - L.poff[hi];
+        rec(d1a[hi][ii * n + jj] + d1b[hi][ii * n + jj], i + 1, j + 1, 0, 0);
+    }
     API_END
 }
 
@@ -1202,7 +1472,7 @@ int v2g_mlp_loss_grad(v2g_handle h, int nb, const float *feat, const float *tgt,
                       double *grad, double *loss)
 {
     API_BEGIN
-    if (!h || !feat || !tgt || !wt || !grad || nb <= 0) throw std::invalid_argument("bad argument");
+    if (!h || !feat || !wt || !grad || nb <= 0) throw std::invalid_argument("bad argument");
     h->ensure_mlp();
     DevBuf<float> part, lpart;
     part.alloc((size_t)v2g::mlp_grad_blocks(nb) * v2g::kMlpParams);
@@ -1214,6 +1484,42 @@ int v2g_mlp_loss_grad(v2g_handle h, int nb, const float *feat, const float *tgt,
     HIPCHK(hipMemcpyAsync(&l, h->dloss.p, sizeof(double), hipMemcpyDeviceToHost, h->st));
     HIPCHK(hipStreamSynchronize(h->st));
     if (loss) *loss = l;
+    API_END
+}
+
+int v2g_mlp_spectral_grad(v2g_handle h, const double *rhs, const double *y, const double *rhi,
+                          const double *rlo, float *dout, double *objective)
+{
+    API_BEGIN
+    if (!h || !rhs || !y || !rhi || !rlo || !dout) throw std::invalid_argument("null argument");
+    h->ensure_mlp();
+    const int64_t n = h->L->ncon;
+    if (n > (int64_t)1 << 30) throw std::invalid_argument("too many rows for one batch");
+    hipStream_t st = h->st;
+    HIPCHK(hipMemcpyAsync(h->yprev.p, y, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    h->mlp_infer(rhs, h->yprev.p);
+    auto rho_terms = [&](const double *r, double *kdv, double &a, double &bq) {
+        v2g::vec_mul(n, h->lminv.p, r, h->vtmp.p, st);
+        v2g::launch_atu(h->plan, h->vtmp.p, h->t.p, st);
+        v2g::launch_au(h->plan, h->t.p, kdv, st);
+        a = h->dot(r, h->vtmp.p, n);
+        bq = h->dot(h->vtmp.p, kdv, n);
+    };
+    double ab[4];
+    rho_terms(rhi, h->kd.p, ab[0], ab[1]);
+    rho_terms(rlo, h->kd2.p, ab[2], ab[3]);
+    HIPCHK(hipMemcpyAsync(h->scales.p + 2, ab, sizeof(ab), hipMemcpyHostToDevice, st));
+    std::vector<int64_t> idx(n);
+    for (int64_t i = 0; i < n; i++) idx[i] = i;
+    DevBuf<int64_t> d_idx;
+    DevBuf<float> feat;
+    d_idx.upload(idx);
+    feat.alloc((size_t)n * v2g::kMlpIn);
+    v2g::mlp_batch_spec((int)n, d_idx.p, h->code.p, h->rownorm.p, rhs, h->yprev.p, rhi, h->kd.p, rlo,
+                        h->kd2.p, h->lminv.p, h->scales.p, feat.p, dout, 3.0, 1.0, h->mlp_jacobi_base,
+                        st);
+    HIPCHK(hipStreamSynchronize(st));
+    if (objective) *objective = std::log(ab[1] / ab[0]) - std::log(ab[3] / ab[2]);
     API_END
 }
 

@@ -67,7 +67,7 @@ k_init(int64_t n, const double *__restrict__ b, const double *__restrict__ Ap,
        double *__restrict__ z, double *__restrict__ partial)
 {
     __shared__ double sh[kThreads / 64];
-    double rz = 0.0;
+    double rz = 0.0, rr = 0.0;
     for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * kThreads) {
         const double ri = b[i] - Ap[i];
@@ -76,9 +76,12 @@ k_init(int64_t n, const double *__restrict__ b, const double *__restrict__ Ap,
         if (z) z[i] = zi;
         p[i] = zi;
         rz += ri * zi;
+        rr += ri * ri;
     }
-    const double s = block_sum(rz, sh);
+    double s = block_sum(rz, sh);
     if (threadIdx.x == 0) partial[P_RZ * kRedBlocks + blockIdx.x] = s;
+    s = block_sum(rr, sh);
+    if (threadIdx.x == 0) partial[P_RR * kRedBlocks + blockIdx.x] = s;
 }
 
 __global__ void k_init_fin(double *__restrict__ partial, double *__restrict__ scal, double tol,
@@ -86,7 +89,9 @@ __global__ void k_init_fin(double *__restrict__ partial, double *__restrict__ sc
 {
     __shared__ double sh[kThreads / 64];
     const double rz = all_sum(partial + P_RZ * kRedBlocks, sh);
+    const double rr = all_sum(partial + P_RR * kRedBlocks, sh);
     if (threadIdx.x == 0) {
+        scal[CG_NRM0] = sqrt(rr);
         scal[S_RZ0] = rz;
         scal[S_DONE0] = (maxiter <= 0) ? 1.0 : 0.0;
         scal[S_DONE1] = 0.0;
@@ -207,6 +212,14 @@ k_axpby(int64_t n, double a, const double *x, double b, const double *y, double 
 }
 
 __global__ void __launch_bounds__(kThreads)
+k_fillv(int64_t n, double v, double *out)
+{
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kThreads)
+        out[i] = v;
+}
+
+__global__ void __launch_bounds__(kThreads)
 k_dotp(int64_t n, const double *__restrict__ a, const double *__restrict__ b,
        double *__restrict__ partial)
 {
@@ -283,6 +296,11 @@ void vec_axpby(int64_t n, double a, const double *x, double b, const double *y, 
                hipStream_t st)
 {
     hipLaunchKernelGGL(k_axpby, dim3(kRedBlocks), dim3(kThreads), 0, st, n, a, x, b, y, out);
+}
+
+void vec_fill(int64_t n, double v, double *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_fillv, dim3(kRedBlocks), dim3(kThreads), 0, st, n, v, out);
 }
 
 void vec_dot_partial(int64_t n, const double *a, const double *b, double *partial, hipStream_t st)

@@ -10,8 +10,8 @@
 // workgroup diagonalizes each 64x64 pair sub-matrix in LDS by parallel
 // cyclic Jacobi, and a second kernel applies the pair rotations to A
 // (tile-wise Q_P^T A_PR Q_R on FP64 MFMA) and to V.  Sweeps repeat until a
-// full sweep applies no rotation above tol = 2e-14 ||M||_F (blocks n <= 64:
-// 1e-15 ||M||_F).
+// full sweep applies no rotation above tol = 2e-14 ||M||_F (64 / N times that
+// when the split must converge; blocks n <= 64: 1e-15 ||M||_F).
 //
 // Warm start: the eigenvectors of the previous Update_xz stay on the device;
 // the next call starts from V^T M V, which is nearly diagonal once the BPSDP
@@ -57,7 +57,7 @@ __device__ __forceinline__ void jacobi_rotation(double app, double aqq, double a
         t = (d >= 0.0 ? e : -e) / (fabs(d) + r);
     } else {   // d^2 + e^2 underflows: the textbook form
         const double th = d / e;
-        t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+        t = (th >= 0.0 ? 1.0 : -1.0) / (fabs(th) + sqrt(fma(th, th, 1.0)));
     }
     const double x = fma(t, t, 1.0), hx = 0.5 * x;
     double y = __builtin_amdgcn_rsq(x);
@@ -65,6 +65,24 @@ __device__ __forceinline__ void jacobi_rotation(double app, double aqq, double a
     y = fma(y, fma(-hx * y, y, 0.5), y);
     c = y;
     s = t * y;
+}
+
+// R_P^T B R_R for one 2 x 2 block B = [b00 b01; b10 b11] with R = [c s; -s c]
+// (rows by (cp, sp), columns by (cr, sr)).  Every product and sum is spelled
+// out with fma(), so the rounding does not depend on how the compiler
+// schedules a particular kernel: each launch shape of the eigen split makes
+// the same rotations bit for bit.
+__device__ __forceinline__ void rot2x2(double cp, double sp, double cr, double sr, double b00,
+                                       double b01, double b10, double b11, double (&o)[4])
+{
+    // rows: R_P^T B
+    const double u00 = fma(cp, b00, -(sp * b10)), u01 = fma(cp, b01, -(sp * b11));
+    const double u10 = fma(sp, b00, cp * b10), u11 = fma(sp, b01, cp * b11);
+    // columns: (R_P^T B) R_R
+    o[0] = fma(cr, u00, -(sr * u01));
+    o[1] = fma(sr, u00, cr * u01);
+    o[2] = fma(cr, u10, -(sr * u11));
+    o[3] = fma(sr, u10, cr * u11);
 }
 
 // parallel cyclic Jacobi on the leading m x m part (m even, m <= kP) of the
@@ -105,14 +123,12 @@ __device__ int lds_jacobi(double *S, double *Q, int m, double tol, int maxsweeps
                 const int p0 = sh_pa[P], p1 = sh_pb[P], r0 = sh_pa[R], r1 = sh_pb[R];
                 const double b00 = S[p0 * kLd + r0], b01 = S[p0 * kLd + r1];
                 const double b10 = S[p1 * kLd + r0], b11 = S[p1 * kLd + r1];
-                // rows: R_P^T B
-                const double u00 = cp * b00 - sp * b10, u01 = cp * b01 - sp * b11;
-                const double u10 = sp * b00 + cp * b10, u11 = sp * b01 + cp * b11;
-                // columns: (R_P^T B) R_R
-                S[p0 * kLd + r0] = cr * u00 - sr * u01;
-                S[p0 * kLd + r1] = sr * u00 + cr * u01;
-                S[p1 * kLd + r0] = cr * u10 - sr * u11;
-                S[p1 * kLd + r1] = sr * u10 + cr * u11;
+                double o[4];
+                rot2x2(cp, sp, cr, sr, b00, b01, b10, b11, o);
+                S[p0 * kLd + r0] = o[0];
+                S[p0 * kLd + r1] = o[1];
+                S[p1 * kLd + r0] = o[2];
+                S[p1 * kLd + r1] = o[3];
             }
             // Q <- Q R on column pairs (rows beyond m are zero in these columns)
             for (int w = threadIdx.x; w < m * h; w += kT) {
@@ -121,8 +137,8 @@ __device__ int lds_jacobi(double *S, double *Q, int m, double tol, int maxsweeps
                 if (sr == 0.0) continue;
                 const int r0 = sh_pa[R], r1 = sh_pb[R];
                 const double v0 = Q[row * kLd + r0], v1 = Q[row * kLd + r1];
-                Q[row * kLd + r0] = cr * v0 - sr * v1;
-                Q[row * kLd + r1] = sr * v0 + cr * v1;
+                Q[row * kLd + r0] = fma(cr, v0, -(sr * v1));
+                Q[row * kLd + r1] = fma(sr, v0, cr * v1);
             }
             __syncthreads();
         }
@@ -180,10 +196,23 @@ template <int NT> struct PairShape {
 // Per step, wave 0 publishes each slot's rotation (sh_c, sh_s) and index pair
 // (sh_pa < sh_pb) for the S update, and each lane's own signed rotation
 // (sh_cl, sh_sl) for the Q update, so no other wave recomputes circle positions.
+// Cross-pair order (bip != 0): step r pairs lane l < 32 (column block a)
+// with lane 32 + ((l + r) mod 32) (column block b), so 32 steps rotate every
+// pair with one index in each block once and no pair inside a block.  A
+// tracking sweep uses it after its first round: that round's cyclic order has
+// already rotated the pairs inside every column block (each block takes part
+// in every round), so the sweep still visits every pair, in 32 + 31 fewer
+// dependent steps per later round.
+__device__ __forceinline__ int cross_partner(int r, int l)
+{
+    return l < kP / 2 ? kP / 2 + ((l + r) & (kP / 2 - 1)) : ((l - kP / 2 - r) & (kP / 2 - 1));
+}
+
 template <int NT>
 __device__ int pair_jacobi(double *S, double (&q)[PairShape<NT>::kRowsPerWave], double tol,
                            int maxsweeps, const unsigned short *ut, int *sh_cnt, double *sh_c,
-                           double *sh_s, int *sh_pa, int *sh_pb, double *sh_cl, double *sh_sl)
+                           double *sh_s, int *sh_pa, int *sh_pb, double *sh_cl, double *sh_sl,
+                           int bip)
 {
     constexpr int kRowsPerWave = PairShape<NT>::kRowsPerWave, kUTper = PairShape<NT>::kUTper;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -199,8 +228,9 @@ __device__ int pair_jacobi(double *S, double (&q)[PairShape<NT>::kRowsPerWave], 
     for (int sweep = 0; sweep < maxsweeps; sweep++) {
         if (threadIdx.x == 0) *sh_cnt = 0;
         __syncthreads();
-        for (int r = 0; r < kP - 1; r++) {
-            const int pl = circle_partner(r, lane);
+        const int nsteps = bip ? kP / 2 : kP - 1;
+        for (int r = 0; r < nsteps; r++) {
+            const int pl = bip ? cross_partner(r, lane) : circle_partner(r, lane);
             if (wave == 0) {
                 const int p = lane < pl ? lane : pl, qq = lane < pl ? pl : lane;
                 bool rotated = false;
@@ -212,7 +242,7 @@ __device__ int pair_jacobi(double *S, double (&q)[PairShape<NT>::kRowsPerWave], 
                         jacobi_rotation(app, aqq, apq, c, sn);
                         rotated = true;
                     }
-                    const int k = circle_slot(r, lane);
+                    const int k = bip ? p : circle_slot(r, lane);
                     sh_c[k] = c;
                     sh_s[k] = sn;
                     sh_pa[k] = p;
@@ -263,12 +293,10 @@ __device__ int pair_jacobi(double *S, double (&q)[PairShape<NT>::kRowsPerWave], 
                     if (act[u]) {
                         const double cp = cf[u][0], sp = cf[u][1], cr = cf[u][2], sr = cf[u][3];
                         const double b00 = bv[u][0], b01 = bv[u][1], b10 = bv[u][2], b11 = bv[u][3];
-                        const double u00 = cp * b00 - sp * b10, u01 = cp * b01 - sp * b11;
-                        const double u10 = sp * b00 + cp * b10, u11 = sp * b01 + cp * b11;
-                        S[addr[u][0]] = cr * u00 - sr * u01;
-                        S[addr[u][1]] = sr * u00 + cr * u01;
-                        S[addr[u][2]] = cr * u10 - sr * u11;
-                        S[addr[u][3]] = sr * u10 + cr * u11;
+                        double o[4];
+                        rot2x2(cp, sp, cr, sr, b00, b01, b10, b11, o);
+#pragma unroll
+                        for (int e = 0; e < 4; e++) S[addr[u][e]] = o[e];
                     }
             }
             // Q <- Q R in registers; every lane takes part in the shuffles
@@ -280,7 +308,7 @@ __device__ int pair_jacobi(double *S, double (&q)[PairShape<NT>::kRowsPerWave], 
 #pragma unroll
                 for (int i = 0; i < kRowsPerWave; i++) o[i] = __shfl(q[i], pl, 64);
 #pragma unroll
-                for (int i = 0; i < kRowsPerWave; i++) q[i] = c * q[i] + sn * o[i];
+                for (int i = 0; i < kRowsPerWave; i++) q[i] = fma(c, q[i], sn * o[i]);
             }
             __syncthreads();
         }
@@ -430,13 +458,18 @@ k_fro_part(const EigBlock *__restrict__ blocks, const double *__restrict__ Mw,
     }
 }
 
-__global__ void k_fro_fin(int nblocks, const double *__restrict__ part, double *__restrict__ tol)
+// rotation threshold of block b: 2e-14 ||M||_F, scaled by 64 / N for a split
+// that must converge (its off-diagonal remainder then stays at the level of
+// one 64-wide block's however large the block is)
+__global__ void k_fro_fin(const EigBlock *__restrict__ blocks, int nblocks,
+                          const double *__restrict__ part, double *__restrict__ tol, int tight)
 {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= nblocks) return;
     double t = 0.0;
     for (int i = 0; i < kFroParts; i++) t += part[b * kFroParts + i];
-    tol[b] = 2e-14 * sqrt(t) + 1e-300;
+    const double scale = tight ? 64.0 / (double)blocks[b].N : 1.0;
+    tol[b] = 2e-14 * scale * sqrt(t) + 1e-300;
 }
 
 // index of the 64 pair rows: first column block a (32 indices), then b
@@ -449,7 +482,7 @@ template <int NT>
 __global__ void __launch_bounds__(NT)
 k_pair_eig(const EigBlock *__restrict__ blocks, int round, const double *__restrict__ Aw,
            double *__restrict__ Qbuf, int *__restrict__ active, const double *__restrict__ tol,
-           int *__restrict__ counter, int inner_sweeps)
+           int *__restrict__ counter, int inner_sweeps, int bip)
 {
     __shared__ double S[kP * kLd];
     __shared__ double sh_c[kP / 2], sh_s[kP / 2];
@@ -502,7 +535,7 @@ k_pair_eig(const EigBlock *__restrict__ blocks, int round, const double *__restr
 #pragma unroll
     for (int i = 0; i < kRowsPerWave; i++) q[i] = (wave * kRowsPerWave + i == lane) ? 1.0 : 0.0;
     const int rot = pair_jacobi<NT>(S, q, tl, inner_sweeps, ut, &sh_cnt, sh_c, sh_s, sh_pa, sh_pb,
-                                    sh_cl, sh_sl);
+                                    sh_cl, sh_sl, bip);
     double *Qo = Qbuf + B.qoff + (int64_t)k * kP * kP;
 #pragma unroll
     for (int i = 0; i < kRowsPerWave; i++) Qo[(wave * kRowsPerWave + i) * kP + lane] = q[i];
@@ -678,22 +711,23 @@ void eig_build_m(const EigBlock *blocks, int nblocks, int max_N, const double *a
 }
 
 void eig_tol(const EigBlock *blocks, int nblocks, const double *Mw, double *tol, double *part,
-             hipStream_t st)
+             int tight, hipStream_t st)
 {
     hipLaunchKernelGGL(k_fro_part, dim3(kFroParts, nblocks), dim3(kT), 0, st, blocks, Mw, part);
-    hipLaunchKernelGGL(k_fro_fin, dim3((nblocks + 63) / 64), dim3(64), 0, st, nblocks, part, tol);
+    hipLaunchKernelGGL(k_fro_fin, dim3((nblocks + 63) / 64), dim3(64), 0, st, blocks, nblocks, part,
+                       tol, tight);
 }
 
 void eig_pairs(const EigBlock *blocks, int nblocks, int maxpairs, int round, const double *Aw,
                double *Qbuf, int *active, const double *tol, int *counter, int inner_sweeps,
-               int wide, hipStream_t st)
+               int wide, int bip, hipStream_t st)
 {
     if (wide)
         hipLaunchKernelGGL(k_pair_eig<1024>, dim3(maxpairs, nblocks), dim3(1024), 0, st, blocks, round,
-                           Aw, Qbuf, active, tol, counter, inner_sweeps);
+                           Aw, Qbuf, active, tol, counter, inner_sweeps, bip);
     else
         hipLaunchKernelGGL(k_pair_eig<256>, dim3(maxpairs, nblocks), dim3(256), 0, st, blocks, round,
-                           Aw, Qbuf, active, tol, counter, inner_sweeps);
+                           Aw, Qbuf, active, tol, counter, inner_sweeps, bip);
 }
 
 void eig_apply(const EigBlock *blocks, int nblocks, int64_t maxtiles, int round, double *Aw,

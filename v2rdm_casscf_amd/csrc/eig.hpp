@@ -32,15 +32,16 @@ void eig_small(const EigBlock *blocks, int nblocks, const double *aty, const dou
 void eig_build_m(const EigBlock *blocks, int nblocks, int max_N, const double *aty, const double *c,
                  const double *x, double mu, double *Mw, double *Aw, double *Vw, int cold,
                  hipStream_t st);
-// tol[b] = 2e-14 ||M_b||_F; part holds 64 * nblocks doubles
+// tol[b] = 2e-14 ||M_b||_F (times 64 / N_b when tight); part holds 64 * nblocks doubles
 void eig_tol(const EigBlock *blocks, int nblocks, const double *Mw, double *tol, double *part,
-             hipStream_t st);
+             int tight, hipStream_t st);
 // one block-Jacobi round: the pair sub-problems (inner_sweeps caps the cyclic
 // sweeps of each 64 x 64 pair; wide = 1: 1024-thread workgroups, for problems
-// whose rounds are latency-bound), then the rotation of the A and V tiles
+// whose rounds are latency-bound; bip = 1: only the pairs across the two
+// column blocks, in 32 steps), then the rotation of the A and V tiles
 void eig_pairs(const EigBlock *blocks, int nblocks, int maxpairs, int round, const double *Aw,
                double *Qbuf, int *active, const double *tol, int *counter, int inner_sweeps,
-               int wide, hipStream_t st);
+               int wide, int bip, hipStream_t st);
 void eig_apply(const EigBlock *blocks, int nblocks, int64_t maxtiles, int round, double *Aw,
                double *Vw, const double *Qbuf, const int *active, hipStream_t st);
 void eig_scale(const EigBlock *blocks, int nblocks, const double *Aw, const double *Vw, double *Tw,

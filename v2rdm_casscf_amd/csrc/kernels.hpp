@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "devlayout.hpp"
 
@@ -24,9 +25,16 @@ struct MapPlan {
     const int64_t *t2_tile_start = nullptr;
     int nt2segs = 0;
     int64_t t2_tiles = 0;
+    // precomputed (D2 column << 1 | sign, D1 column) of the G2 rows, indexed
+    // by each G2 row segment's aux + visiting position (null: not built)
+    const int2 *g2idx = nullptr;
 };
 
 int64_t tiles_for(const Segment &s);
+// fill the G2 column table for the row segments whose aux is >= 0; D1 columns
+// of x are [d1lo, d1hi)
+void build_g2_index(const MapPlan &p, const std::vector<Segment> &rowsegs, int64_t d1lo,
+                    int64_t d1hi, int2 *idx, hipStream_t st);
 // dotv / rowpart (optional): rowpart[tile] = sum over the tile's rows of dotv . Ax
 void launch_au(const MapPlan &p, const double *x, double *Ax, hipStream_t st,
                const double *done = nullptr, const double *dotv = nullptr,
@@ -42,6 +50,7 @@ enum CgSlot : int {
     CG_RZ0 = 16, CG_RZ1 = 17, CG_DONE0 = 18, CG_DONE1 = 19, CG_NRM = 20, CG_ITER = 21,
     CG_ALPHA = 22, CG_BETA = 23, CG_PAP = 24, CG_TOL = 25, CG_MAXIT = 26,
     CG_IT0 = 27, CG_IT1 = 28,   // iteration counter, ping-pong like r.z
+    CG_NRM0 = 29,               // ||r|| of the starting guess
     CG_NSLOTS = 32
 };
 constexpr int kRedBlocks = 2048;    // partial-sum slots per reduction (= grid of the vector kernels)
@@ -64,6 +73,7 @@ void cg_finish_iter(int64_t n, double *y, double *r, double *p, const double *z,
 // ---- vector helpers (cg.hip) ----
 void vec_axpby(int64_t n, double a, const double *x, double b, const double *y, double *out,
                hipStream_t st);
+void vec_fill(int64_t n, double v, double *out, hipStream_t st);
 void vec_dot_partial(int64_t n, const double *a, const double *b, double *partial, hipStream_t st);
 void vec_reduce(const double *partial, double *out, hipStream_t st);
 // y <- 2 y - yh and yh <- old y (linear extrapolation of the CG starting guess)

@@ -23,6 +23,9 @@
 // g2.cc:1061-1260 and t2.cc:1251-1861.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <vector>
+
 #include "devlayout.hpp"
 #include "kernels.hpp"
 #include "rowterms.hpp"
@@ -102,11 +105,41 @@ __device__ __forceinline__ int find_segment(const int64_t *__restrict__ tile_sta
     return lo;
 }
 
+// Column indices of the G2 rows (65 % of all rows at large active spaces),
+// built once per layout: for the row visited at position pos of a G2
+// segment, g2idx[aux + pos] = (D2 column << 1 | coefficient sign, or -1 when
+// the row has no D2 term; D1 column or -1).  A.x then reads one coalesced
+// int2 instead of walking the pair and geminal tables before its D2 gather.
+struct G2Rec {
+    int64_t d1lo, d1hi;     // D1a / D1b columns of x
+    int64_t n = 0, c2 = -1, c1 = -1;
+    __device__ __forceinline__ void operator()(int64_t col, double coef)
+    {
+        if (n++ == 0) return;                       // the G2 entry itself
+        if (col >= d1lo && col < d1hi) c1 = col;    // delta-selected D1 term (+1)
+        else c2 = (col << 1) | (coef < 0.0 ? 1 : 0);
+    }
+};
+
+__global__ void __launch_bounds__(kBlock)
+k_g2_index(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs, int si, int64_t d1lo,
+           int64_t d1hi, int2 *__restrict__ idx)
+{
+    const DevLayout &L = *Lp;
+    const Segment sg = segs[si];
+    for (int64_t pos = blockIdx.x * (int64_t)kBlock + threadIdx.x; pos < sg.count;
+         pos += (int64_t)gridDim.x * kBlock) {
+        G2Rec f{d1lo, d1hi};
+        elem_terms(L, sg, row_of(L, sg, pos), f);
+        idx[sg.aux + pos] = make_int2((int)f.c2, (int)f.c1);
+    }
+}
+
 __global__ void __launch_bounds__(kBlock)
 k_au(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
      const int64_t *__restrict__ tile_start, int nseg, const double *__restrict__ x,
      double *__restrict__ Ax, const double *__restrict__ done, const double *__restrict__ dotv,
-     double *__restrict__ rowpart)
+     double *__restrict__ rowpart, const int2 *__restrict__ g2idx)
 {
     if (done && *done != 0.0) return;
     __shared__ double red[kWavesPerBlock];
@@ -126,6 +159,42 @@ k_au(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
             if (lane == 0) {
                 Ax[sg.start + item] = v;
                 if (dotv) contrib = v * dotv[sg.start + item];
+            }
+        }
+    } else if (g2idx && sg.aux >= 0) {
+        // G2 rows: self entry, D2 term and D1 term from the precomputed columns
+        const int64_t sb = sg.kind == RF_G2AA ? L.xoff[XB_G2AA][sg.h]
+                                              : L.xoff[sg.sub ? XB_G2BA : XB_G2AB][sg.h];
+        double v[kPerThread];
+        int64_t row[kPerThread];
+        int2 ci[kPerThread];
+#pragma unroll
+        for (int u = 0; u < kPerThread; u++) {
+            const int64_t pos = t * kElemTile + u * kBlock + threadIdx.x;
+            row[u] = -1;
+            if (pos < sg.count) {
+                ci[u] = g2idx[sg.aux + pos];
+                row[u] = row_of(L, sg, pos);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kPerThread; u++) {
+            v[u] = 0.0;
+            if (row[u] >= 0) {
+                double s = -x[sb + row[u]];
+                if (ci[u].x >= 0) {
+                    const double d2 = x[ci[u].x >> 1];
+                    s += (ci[u].x & 1) ? -d2 : d2;
+                }
+                if (ci[u].y >= 0) s += x[ci[u].y];
+                v[u] = s;
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kPerThread; u++) {
+            if (row[u] >= 0) {
+                Ax[sg.start + row[u]] = v[u];
+                if (dotv) contrib += v[u] * dotv[sg.start + row[u]];
             }
         }
     } else {
@@ -275,7 +344,20 @@ void launch_au(const MapPlan &p, const double *x, double *Ax, hipStream_t st, co
 {
     if (p.row_tiles > 0)
         hipLaunchKernelGGL(k_au, dim3((unsigned)p.row_tiles), dim3(kBlock), 0, st, p.dlay,
-                           p.rowsegs, p.row_tile_start, p.nrowsegs, x, Ax, done, dotv, rowpart);
+                           p.rowsegs, p.row_tile_start, p.nrowsegs, x, Ax, done, dotv, rowpart,
+                           p.g2idx);
+}
+
+void build_g2_index(const MapPlan &p, const std::vector<Segment> &rowsegs, int64_t d1lo,
+                    int64_t d1hi, int2 *idx, hipStream_t st)
+{
+    for (size_t si = 0; si < rowsegs.size(); si++) {
+        const Segment &s = rowsegs[si];
+        if (s.aux < 0 || s.count == 0) continue;
+        const unsigned g = (unsigned)std::min<int64_t>(4096, (s.count + kBlock - 1) / kBlock);
+        hipLaunchKernelGGL(k_g2_index, dim3(g), dim3(kBlock), 0, st, p.dlay, p.rowsegs, (int)si, d1lo,
+                           d1hi, idx);
+    }
 }
 
 void launch_row_norm2(const MapPlan &p, double *d, hipStream_t st)

@@ -1,8 +1,9 @@
 // mlp.hip -- the learned diagonal preconditioner of the CG on A A^T.
 //
 // A per-constraint-row MLP (14 inputs, two tanh layers of 64, one output g,
-// FP32 weights) rescales the Jacobi diagonal: M^-1_ii = exp(clamp(g_i)) /
-// diag(A A^T)_ii, which keeps the preconditioner SPD for any weights.
+// FP32 weights) gives the diagonal M^-1_ii = exp(clamp(g_i)) (identity base:
+// g = 0 is the reference's plain CG) or exp(clamp(g_i)) / diag(A A^T)_ii
+// (Jacobi base), SPD for any weights.
 // Inputs per row: one-hot constraint family (11), log diag(A A^T)_ii / 4,
 // asinh(rhs_i / rms(rhs)), asinh(y_i / rms(y)) with y the previous solution.
 //
@@ -88,7 +89,7 @@ __global__ void __launch_bounds__(kT)
 k_mlp_minv(int64_t n, const unsigned char *__restrict__ code, const double *__restrict__ d,
            const double *__restrict__ rhs, const double *__restrict__ y,
            const float *__restrict__ Wg, const double *__restrict__ scales,
-           double *__restrict__ minv, double clampv)
+           double *__restrict__ minv, double clampv, int jacobi_base)
 {
     __shared__ float W[kMlpParams];
     for (int i = threadIdx.x; i < kMlpParams; i += kT) W[i] = Wg[i];
@@ -100,19 +101,28 @@ k_mlp_minv(int64_t n, const unsigned char *__restrict__ code, const double *__re
         features(code[i], d[i], rhs[i], y[i], rs, ys, f);
         float g = forward(W, f);
         g = fminf(fmaxf(g, (float)-clampv), (float)clampv);
-        const double di = d[i] > 0.0 ? d[i] : 1.0;
+        const double di = (jacobi_base && d[i] > 0.0) ? d[i] : 1.0;
         minv[i] = exp((double)g) / di;
     }
 }
 
-// batch features / targets from a CG solve: r0 = rhs - A A^T y_prev and
-// dy = y_new - y_prev; target log(dy d / r0) clamped, weight (r0/d)^2
+// batch for the spectral objective L = log rho(r_hi) - log rho(r_lo), with
+// rho(r) = (Sr).K(Sr) / (r.Sr) the Rayleigh quotient of the preconditioned
+// operator S K at z = S r: r_hi leans towards its largest eigenvalues (power
+// iterations), r_lo towards its smallest (the final residual of a CG solve),
+// so L estimates the log condition number.  The upstream gradient with
+// respect to g_i (s_i = exp(g_i) / base_i) is
+//   dL/dg_i = [2 s_i r_i (K S r)_i / b - s_i r_i^2 / a]_hi - [ ... ]_lo
+// with a = r.Sr and b = (Sr).K(Sr) (scales[2..5] = a_hi, b_hi, a_lo, b_lo);
+// zero where g is clamped.  `coef` = ncon / nb.
 __global__ void __launch_bounds__(kT)
-k_mlp_batch(int nb, const int64_t *__restrict__ idx, const unsigned char *__restrict__ code,
-            const double *__restrict__ d, const double *__restrict__ rhs,
-            const double *__restrict__ yprev, const double *__restrict__ r0,
-            const double *__restrict__ dy, const double *__restrict__ scales, float *__restrict__ feat,
-            float *__restrict__ tgt, float *__restrict__ wt, double clampv)
+k_mlp_batch_spec(int nb, const int64_t *__restrict__ idx, const unsigned char *__restrict__ code,
+                 const double *__restrict__ d, const double *__restrict__ rhs,
+                 const double *__restrict__ yprev, const double *__restrict__ rhi,
+                 const double *__restrict__ kdhi, const double *__restrict__ rlo,
+                 const double *__restrict__ kdlo, const double *__restrict__ s,
+                 const double *__restrict__ scales, float *__restrict__ feat,
+                 float *__restrict__ dout, double clampv, double coef, int jacobi_base)
 {
     const int b = blockIdx.x * kT + threadIdx.x;
     if (b >= nb) return;
@@ -120,40 +130,15 @@ k_mlp_batch(int nb, const int64_t *__restrict__ idx, const unsigned char *__rest
     float f[kMlpIn];
     features(code[i], d[i], rhs[i], yprev[i], (float)scales[0], (float)scales[1], f);
     for (int k = 0; k < kMlpIn; k++) feat[(int64_t)b * kMlpIn + k] = f[k];
-    const double di = d[i] > 0.0 ? d[i] : 1.0;
-    const double step = r0[i] / di;
-    const double rho = (step != 0.0) ? dy[i] / step : 0.0;
-    double t;
-    if (rho > 0.0) t = fmin(fmax(log(rho), -clampv), clampv);
-    else t = -clampv;
-    tgt[b] = (float)t;
-    wt[b] = (float)(step * step * scales[2]);
-}
-
-// batch for the step-quality objective J = (r.Sr)^2 / ((Sr).K(Sr)), S = diag(s):
-// features of the sampled rows and the upstream gradient of -log J with
-// respect to the network output g_i (s_i = exp(g_i) / d_i), zero where g is
-// clamped; `coef` = (ncon / nb) so the batch sum estimates the full sum
-__global__ void __launch_bounds__(kT)
-k_mlp_batch_j(int nb, const int64_t *__restrict__ idx, const unsigned char *__restrict__ code,
-              const double *__restrict__ d, const double *__restrict__ rhs,
-              const double *__restrict__ yprev, const double *__restrict__ r,
-              const double *__restrict__ s, const double *__restrict__ kd,
-              const double *__restrict__ scales, float *__restrict__ feat,
-              float *__restrict__ dout, double clampv, double coef)
-{
-    const int b = blockIdx.x * kT + threadIdx.x;
-    if (b >= nb) return;
-    const int64_t i = idx[b];
-    float f[kMlpIn];
-    features(code[i], d[i], rhs[i], yprev[i], (float)scales[0], (float)scales[1], f);
-    for (int k = 0; k < kMlpIn; k++) feat[(int64_t)b * kMlpIn + k] = f[k];
-    const double a = scales[2], bq = scales[3];
-    const double di = d[i] > 0.0 ? d[i] : 1.0;
-    const double g = log(s[i] * di);
+    const double ahi = scales[2], bhi = scales[3], alo = scales[4], blo = scales[5];
+    const double base = (jacobi_base && d[i] > 0.0) ? d[i] : 1.0;
+    const double g = log(s[i] * base);
     double u = 0.0;
-    if (fabs(g) < clampv - 1e-6 && a != 0.0 && bq != 0.0)
-        u = -s[i] * (2.0 * r[i] * r[i] / a - 2.0 * kd[i] * r[i] / bq);
+    if (fabs(g) < clampv - 1e-6 && ahi > 0.0 && bhi > 0.0 && alo > 0.0 && blo > 0.0) {
+        const double uh = 2.0 * s[i] * rhi[i] * kdhi[i] / bhi - s[i] * rhi[i] * rhi[i] / ahi;
+        const double ul = 2.0 * s[i] * rlo[i] * kdlo[i] / blo - s[i] * rlo[i] * rlo[i] / alo;
+        u = uh - ul;
+    }
     dout[b] = (float)(coef * u);
 }
 
@@ -335,28 +320,21 @@ void mlp_family_codes(const Segment *segs_dev, const std::vector<Segment> &segs_
 
 void mlp_minv(int64_t n, const unsigned char *code, const double *d, const double *rhs,
               const double *y, const float *W, const double *scales, double *minv, double clampv,
-              hipStream_t st)
+              int jacobi_base, hipStream_t st)
 {
     hipLaunchKernelGGL(k_mlp_minv, dim3(2048), dim3(kT), 0, st, n, code, d, rhs, y, W, scales, minv,
-                       clampv);
+                       clampv, jacobi_base);
 }
 
-void mlp_batch(int nb, const int64_t *idx, const unsigned char *code, const double *d,
-               const double *rhs, const double *yprev, const double *r0, const double *dy,
-               const double *scales, float *feat, float *tgt, float *wt, double clampv,
-               hipStream_t st)
+void mlp_batch_spec(int nb, const int64_t *idx, const unsigned char *code, const double *d,
+                    const double *rhs, const double *yprev, const double *rhi, const double *kdhi,
+                    const double *rlo, const double *kdlo, const double *s, const double *scales,
+                    float *feat, float *dout, double clampv, double coef, int jacobi_base,
+                    hipStream_t st)
 {
-    hipLaunchKernelGGL(k_mlp_batch, dim3((nb + kT - 1) / kT), dim3(kT), 0, st, nb, idx, code, d,
-                       rhs, yprev, r0, dy, scales, feat, tgt, wt, clampv);
-}
-
-void mlp_batch_j(int nb, const int64_t *idx, const unsigned char *code, const double *d,
-                 const double *rhs, const double *yprev, const double *r, const double *s,
-                 const double *kd, const double *scales, float *feat, float *dout, double clampv,
-                 double coef, hipStream_t st)
-{
-    hipLaunchKernelGGL(k_mlp_batch_j, dim3((nb + kT - 1) / kT), dim3(kT), 0, st, nb, idx, code, d,
-                       rhs, yprev, r, s, kd, scales, feat, dout, clampv, coef);
+    hipLaunchKernelGGL(k_mlp_batch_spec, dim3((nb + kT - 1) / kT), dim3(kT), 0, st, nb, idx, code, d,
+                       rhs, yprev, rhi, kdhi, rlo, kdlo, s, scales, feat, dout, clampv, coef,
+                       jacobi_base);
 }
 
 int mlp_grad_blocks(int nb) { return (nb + kS - 1) / kS; }

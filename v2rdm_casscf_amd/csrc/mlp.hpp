@@ -19,18 +19,16 @@ constexpr int kMlpParams = kMlpIn * kMlpH + kMlpH + kMlpH * kMlpH + kMlpH + kMlp
 
 void mlp_family_codes(const Segment *segs_dev, const std::vector<Segment> &segs_host,
                       unsigned char *code, hipStream_t st);
+// minv = exp(clamp(g)), divided by d when jacobi_base
 void mlp_minv(int64_t n, const unsigned char *code, const double *d, const double *rhs,
               const double *y, const float *W, const double *scales, double *minv, double clampv,
-              hipStream_t st);
-void mlp_batch(int nb, const int64_t *idx, const unsigned char *code, const double *d,
-               const double *rhs, const double *yprev, const double *r0, const double *dy,
-               const double *scales, float *feat, float *tgt, float *wt, double clampv,
-               hipStream_t st);
-// scales[2] = a = r.Sr, scales[3] = b = (Sr).K(Sr)
-void mlp_batch_j(int nb, const int64_t *idx, const unsigned char *code, const double *d,
-                 const double *rhs, const double *yprev, const double *r, const double *s,
-                 const double *kd, const double *scales, float *feat, float *dout, double clampv,
-                 double coef, hipStream_t st);
+              int jacobi_base, hipStream_t st);
+// spectral objective: scales[2..5] = a_hi, b_hi, a_lo, b_lo
+void mlp_batch_spec(int nb, const int64_t *idx, const unsigned char *code, const double *d,
+                    const double *rhs, const double *yprev, const double *rhi, const double *kdhi,
+                    const double *rlo, const double *kdlo, const double *s, const double *scales,
+                    float *feat, float *dout, double clampv, double coef, int jacobi_base,
+                    hipStream_t st);
 int mlp_grad_blocks(int nb);
 // tgt == nullptr: wt holds the upstream gradient d(loss)/d(out) per sample
 void mlp_grad(int nb, const float *feat, const float *tgt, const float *wt, const float *W,

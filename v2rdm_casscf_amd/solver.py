@@ -81,7 +81,7 @@ class V2RDMSolver:
                             MU_UPDATE_FREQUENCY=500, PRECONDITIONER="NONE", FLEXIBLE_CG=False,
                             MLP_TRAIN_ITERS=30, MLP_TRAIN_BATCH=16384, MLP_TRAIN_STEPS=10,
                             MLP_TRAIN_LR=2e-3, EIG_WARM_SWEEPS=1, EIG_INNER_SWEEPS=1,
-                            CG_EXTRAPOLATE=True)
+                            CG_EXTRAPOLATE=True, MLP_BASE="IDENTITY", EIG_PAIR_ORDER=1)
         if options:
             self.options.update({k.upper(): v for k, v in options.items()})
         self.x = self.z = self.y = self.c = self.b = None
@@ -173,15 +173,26 @@ class V2RDMSolver:
         check(self.lib.v2g_mlp_set_weights(self.h, _np_ptr(w)))
 
     def mlp_loss_grad(self, feat, tgt, wt):
-        """feat: (nb, 14) float32, tgt/wt: (nb,) float32 device tensors."""
+        """feat: (nb, 14) float32, tgt/wt: (nb,) float32 device tensors; with
+        tgt None, wt is the upstream gradient d(loss)/d(out) per sample."""
         nb = feat.shape[0]
         g = torch.empty(self.mlp_num_params(), dtype=torch.float64, device=self.device)
         loss = ctypes.c_double()
+        feat, wt = feat.contiguous(), wt.contiguous()
+        tp = _ptr(tgt.contiguous()) if tgt is not None else None
         self._sync()
-        check(self.lib.v2g_mlp_loss_grad(self.h, int(nb), _ptr(feat.contiguous()),
-                                         _ptr(tgt.contiguous()), _ptr(wt.contiguous()), _ptr(g),
+        check(self.lib.v2g_mlp_loss_grad(self.h, int(nb), _ptr(feat), tp, _ptr(wt), _ptr(g),
                                          ctypes.byref(loss)))
         return loss.value, g
+
+    def mlp_spectral_grad(self, rhs, y, r_hi, r_lo):
+        """Per-row d L / d g of the spectral training objective and L."""
+        out = torch.empty(self.nconstraints, dtype=torch.float32, device=self.device)
+        obj = ctypes.c_double()
+        self._sync()
+        check(self.lib.v2g_mlp_spectral_grad(self.h, _ptr(rhs), _ptr(y), _ptr(r_hi), _ptr(r_lo),
+                                             _ptr(out), ctypes.byref(obj)))
+        return out, obj.value
 
     def mlp_adam_step(self, grad, lr):
         check(self.lib.v2g_mlp_adam_step(self.h, _ptr(grad), float(lr)))
@@ -221,24 +232,45 @@ class V2RDMSolver:
         """Time every launch shape on the next warm splits and keep the fastest."""
         check(self.lib.v2g_set_autotune(self.h, int(bool(on))))
 
-    # ---- path inputs (host) -------------------------------------------
-    def BuildConstraints(self):
+    # ---- path inputs ----------------------------------------------------
+    # Built on the device by default (bit-identical to the host builders,
+    # which stay available with on_device=False)
+    def BuildConstraints(self, on_device=True):
+        if on_device:
+            self.b = self._vec(self.nconstraints)
+            self._sync()
+            check(self.lib.v2g_build_b_device(self.h, _ptr(self.b)))
+            return self.b
         b = np.empty(self.nconstraints)
         check(self.lib.v2g_build_b(self.h, _np_ptr(b)))
         self.b = torch.from_numpy(b).to(self.device)
         return self.b
 
-    def GetIntegrals(self, h1, eri):
+    def GetIntegrals(self, h1, eri, on_device=True):
         """c from active-space integrals (h1: amo x amo, eri: amo^4 chemists')."""
         amo = sum(self.amopi)
         h1 = np.ascontiguousarray(h1, dtype=np.float64).reshape(amo, amo)
         eri = np.ascontiguousarray(eri, dtype=np.float64).reshape(amo, amo, amo, amo)
+        if on_device:
+            dh1 = torch.from_numpy(h1).to(self.device)
+            deri = torch.from_numpy(eri).to(self.device)
+            self.c = self._vec(self.dimx)
+            self._sync()
+            check(self.lib.v2g_build_c_device(self.h, _ptr(dh1), _ptr(deri), _ptr(self.c)))
+            return self.c
         c = np.empty(self.dimx)
         check(self.lib.v2g_build_c(self.h, _np_ptr(h1), _np_ptr(eri), _np_ptr(c)))
         self.c = torch.from_numpy(c).to(self.device)
         return self.c
 
-    def Guess(self):
+    def Guess(self, on_device=True):
+        if on_device:
+            self.x, self.z = self._vec(self.dimx), self._vec(self.dimx)
+            self.y = self._vec(self.nconstraints)
+            self._sync()
+            check(self.lib.v2g_guess_random_device(self.h, _ptr(self.x), _ptr(self.z),
+                                                   _ptr(self.y)))
+            return self.x, self.z, self.y
         x = np.empty(self.dimx)
         z = np.empty(self.dimx)
         y = np.empty(self.nconstraints)
@@ -247,6 +279,63 @@ class V2RDMSolver:
         self.z = torch.from_numpy(z).to(self.device)
         self.y = torch.from_numpy(y).to(self.device)
         return self.x, self.z, self.y
+
+    # ---- RDM outputs and checkpoint ---------------------------------------
+    def get_opdm(self):
+        """v2RDMSolver::get_opdm (python_api.cc:376-396): list of per-irrep
+        amopi[h] x amopi[h] numpy arrays, D1a + D1b."""
+        out = np.empty(sum(n * n for n in self.amopi))
+        self._sync()
+        check(self.lib.v2g_get_opdm(self.h, _ptr(self.x), _np_ptr(out)))
+        blocks, o = [], 0
+        for n in self.amopi:
+            blocks.append(out[o:o + n * n].reshape(n, n))
+            o += n * n
+        return blocks
+
+    def get_tpdm(self):
+        """v2RDMSolver::get_tpdm (python_api.cc:398-458): amo^2 x amo^2 numpy
+        array, element [i amo + k, j amo + l]; C1 only."""
+        amo = sum(self.amopi)
+        out = np.empty((amo * amo, amo * amo))
+        self._sync()
+        check(self.lib.v2g_get_tpdm(self.h, _ptr(self.x), _np_ptr(out)))
+        return out
+
+    def write_fcidump_rdm(self, path):
+        """The .rdm file of v2RDMSolver::FCIDUMP (fcidump.cc:176-268)."""
+        self._sync()
+        check(self.lib.v2g_write_fcidump_rdm(self.h, _ptr(self.x), str(path).encode()))
+
+    def write_checkpoint(self, path):
+        """WriteCheckpointFile (checkpoint.cc:56-87): entries MU, PRIMAL (x),
+        DUAL 1 (y), DUAL 2 (z), plus the iteration count and the last errors
+        so that compute_energy(resume=True) continues the solve; a numpy .npz
+        file (no pickled objects) instead of the psio file 269."""
+        if self.x is None:
+            raise ValueError("nothing to write: no solve has run")
+        st = self.last_stats or dict(mu=self.mu, oiter=0, ep=0.0, ed=0.0)
+        with open(path, "wb") as f:
+            np.savez(f, **{"MU": np.float64(st["mu"]), "PRIMAL": self.x.cpu().numpy(),
+                           "DUAL 1": self.y.cpu().numpy(), "DUAL 2": self.z.cpu().numpy(),
+                           "OITER": np.int64(st["oiter"]), "EP": np.float64(st["ep"]),
+                           "ED": np.float64(st["ed"]),
+                           "DIMS": np.array([self.dimx, self.nconstraints], dtype=np.int64)})
+
+    def read_checkpoint(self, path):
+        """ReadFromCheckpointFile (checkpoint.cc:89-121): x, y, z and mu; the
+        next compute_energy(resume=True) continues from them (with a fresh
+        eigen split, as the reference starts one)."""
+        with np.load(path, allow_pickle=False) as d:
+            if tuple(d["DIMS"]) != (self.dimx, self.nconstraints):
+                raise ValueError("checkpoint dimensions do not match this layout")
+            self.x = torch.from_numpy(d["PRIMAL"].copy()).to(self.device)
+            self.y = torch.from_numpy(d["DUAL 1"].copy()).to(self.device)
+            self.z = torch.from_numpy(d["DUAL 2"].copy()).to(self.device)
+            self.mu = float(d["MU"])
+            self.last_stats = dict(mu=self.mu, oiter=int(d["OITER"]), ep=float(d["EP"]),
+                                   ed=float(d["ED"]))
+        self.reset_eig()
 
     # ---- outer loop ---------------------------------------------------
     def set_iter_callback(self, fn):
@@ -290,6 +379,8 @@ class V2RDMSolver:
                         int(o["MLP_TRAIN_BATCH"]), int(o["MLP_TRAIN_STEPS"]),
                         float(o["MLP_TRAIN_LR"]), int(o["EIG_WARM_SWEEPS"]),
                         int(o["EIG_INNER_SWEEPS"]), int(bool(o["CG_EXTRAPOLATE"])))
+        p.mlp_base = {"IDENTITY": 0, "JACOBI": 1}[str(o["MLP_BASE"]).upper()]
+        p.eig_pair_order = int(o["EIG_PAIR_ORDER"])
         prev = self.last_stats if resume else None
         if prev:
             p.mu0, p.oiter0, p.ep0, p.ed0 = prev["mu"], prev["oiter"], prev["ep"], prev["ed"]
