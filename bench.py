@@ -225,11 +225,18 @@ def _scan(args):
         print(f"# point {i}: H{r['n']} r={r['spacing_angstrom']:.3f} E={r['energy']:.8f} "
               f"oiter={r['oiter']} converged={r['converged']}", file=sys.stderr, flush=True)
 
-    res, wall = solve_scan(pts, positivity=args.positivity, workers=len(pts), options=opts,
-                           max_seconds=args.converge, progress=progress)
+    if args.scan_threads:
+        from v2rdm_casscf_amd.bondscan import solve_scan_threads
+        res, wall = solve_scan_threads(pts, positivity=args.positivity, workers=len(pts),
+                                       options=opts, max_seconds=args.converge, progress=progress)
+    else:
+        opts.pop("PRECONDITIONER", None)
+        res, wall = solve_scan(pts, positivity=args.positivity, options=opts,
+                               max_seconds=args.converge, progress=progress)
     conv = sum(int(r["converged"]) for r in res)
     tot = sum(r["oiter"] for r in res)
-    print(json.dumps(dict(mode="bond-scan", points=len(pts), converged=conv, seconds=round(wall, 2),
+    print(json.dumps(dict(mode="bond-scan", engine="threads" if args.scan_threads else "batch",
+                          points=len(pts), converged=conv, seconds=round(wall, 2),
                           outer_iterations=tot, outer_iterations_per_s=round(tot / wall, 1),
                           max_outer=max(r["oiter"] for r in res),
                           per_point=[dict(n=r["n"], r=round(r["spacing_angstrom"], 4),
@@ -271,6 +278,9 @@ def main():
     ap.add_argument("--scan", type=int, default=0,
                     help="batched bond scan: this many H-chains (n in 8..16, spacing 0.7-3.0 A, "
                          "seed 0) solved concurrently to R/E_CONVERGENCE 1e-6; --converge caps it")
+    ap.add_argument("--scan-threads", action="store_true",
+                    help="with --scan: one engine handle and stream per point on host threads "
+                         "instead of one batch")
     ap.add_argument("--cpu-iters", type=int, default=0,
                     help="with --converge: time this many CPU-oracle outer iterations too")
     ap.add_argument("--maps-only", action="store_true",

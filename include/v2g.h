@@ -153,6 +153,9 @@ typedef struct v2g_params {
     int mlp_base;                /* precond 3: 0 the learned diagonal      */
                                  /* scales the identity (plain CG), 1 the  */
                                  /* Jacobi diagonal                        */
+    double train_penalty;        /* precond 3: weight of (1/2) mean(g^2),  */
+                                 /* which holds the learned diagonal near  */
+                                 /* its base                               */
     int eig_pair_order;          /* capped warm splits: 0 every round's    */
                                  /* 64 x 64 pairs in cyclic order; 1 cyclic */
                                  /* in the first round of a sweep, then    */
@@ -221,6 +224,30 @@ int v2g_get_tpdm(v2g_handle h, const double *x_dev, double *tpdm_host);
  * (double value, int i, j, k, l) of the spin-summed 2-RDM, then the 1-RDM
  * with k = l = 0; active orbitals numbered from 1 in Pitzer order (no core) */
 int v2g_write_fcidump_rdm(v2g_handle h, const double *x_dev, const char *path);
+
+/* ---- batches of independent problems (bond scans; no reference
+ * counterpart: the reference solves one problem per process) ----
+ * All problems' vectors are concatenated: problem q's x / z / c entries start
+ * at xoff[q], its y / b rows at yoff[q] (v2g_batch_offsets fills nprob + 1
+ * entries, the last one the totals).  Every kernel launch of the iteration
+ * covers the whole batch; each problem keeps its own mu, errors, CG scalars
+ * and stopping rule, and leaves the batch when it converges. */
+typedef struct v2g_batch *v2g_batch_handle;
+int v2g_batch_create(const v2g_layout_desc *descs, int nprob, v2g_batch_handle *out);
+int v2g_batch_destroy(v2g_batch_handle h);
+int v2g_batch_offsets(v2g_batch_handle h, long long *xoff, long long *yoff);
+/* bpsdp_Au / bpsdp_ATu of every problem at once (concatenated vectors) */
+int v2g_batch_Au(v2g_batch_handle h, const double *x_dev, double *Ax_dev);
+int v2g_batch_ATu(v2g_batch_handle h, const double *y_dev, double *ATy_dev);
+/* b, c (from per-problem device h1 / eri arrays) and the RANDOM guess of
+ * every problem, on the device */
+int v2g_batch_inputs(v2g_batch_handle h, const double *const *h1_dev, const double *const *eri_dev,
+                     double *c_dev, double *b_dev, double *x_dev, double *z_dev, double *y_dev);
+/* the BPSDP loop of every problem (precond 0, the reference's plain CG);
+ * stats receives nprob records */
+int v2g_batch_bpsdp(v2g_batch_handle h, const v2g_params *p, const double *c_dev,
+                    const double *b_dev, double *x_dev, double *z_dev, double *y_dev,
+                    v2g_stats *stats);
 
 const char *v2g_last_error(void);
 const char *v2g_version(void);

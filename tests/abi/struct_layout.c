@@ -26,6 +26,7 @@ int main(void)
     F(v2g_params, eig_warm_sweeps); F(v2g_params, eig_inner_sweeps);
     F(v2g_params, cg_extrapolate); F(v2g_params, mu0); F(v2g_params, oiter0);
     F(v2g_params, ep0); F(v2g_params, ed0); F(v2g_params, mlp_base);
+    F(v2g_params, train_penalty);
     F(v2g_params, eig_pair_order);
     S(v2g_stats);
     F(v2g_stats, oiter); F(v2g_stats, cg_iters); F(v2g_stats, converged);

@@ -32,7 +32,7 @@ class Params(ctypes.Structure):
                 ("eig_inner_sweeps", ctypes.c_int), ("cg_extrapolate", ctypes.c_int),
                 ("mu0", ctypes.c_double), ("oiter0", ctypes.c_int), ("ep0", ctypes.c_double),
                 ("ed0", ctypes.c_double), ("mlp_base", ctypes.c_int),
-                ("eig_pair_order", ctypes.c_int)]
+                ("train_penalty", ctypes.c_double), ("eig_pair_order", ctypes.c_int)]
 
 
 class Stats(ctypes.Structure):
@@ -93,6 +93,17 @@ SIGNATURES = {
     "v2g_get_opdm": (ctypes.c_int, [_VP, _DP, _DP]),
     "v2g_get_tpdm": (ctypes.c_int, [_VP, _DP, _DP]),
     "v2g_write_fcidump_rdm": (ctypes.c_int, [_VP, _DP, ctypes.c_char_p]),
+    "v2g_batch_create": (ctypes.c_int, [ctypes.POINTER(LayoutDesc), ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_void_p)]),
+    "v2g_batch_destroy": (ctypes.c_int, [_VP]),
+    "v2g_batch_offsets": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_longlong),
+                                         ctypes.POINTER(ctypes.c_longlong)]),
+    "v2g_batch_Au": (ctypes.c_int, [_VP, _DP, _DP]),
+    "v2g_batch_ATu": (ctypes.c_int, [_VP, _DP, _DP]),
+    "v2g_batch_inputs": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_void_p),
+                                        ctypes.POINTER(ctypes.c_void_p), _DP, _DP, _DP, _DP, _DP]),
+    "v2g_batch_bpsdp": (ctypes.c_int, [_VP, ctypes.POINTER(Params), _DP, _DP, _DP, _DP, _DP,
+                                       ctypes.POINTER(Stats)]),
     "v2g_last_error": (ctypes.c_char_p, []),
     "v2g_version": (ctypes.c_char_p, []),
 }

@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "../../include/v2g.h"
+#include "common.hpp"
 #include "devlayout.hpp"
 #include "eig.hpp"
 #include "inputs.hpp"
@@ -24,46 +25,19 @@
 
 #include <random>
 
+namespace v2g {
+std::string &last_error()
+{
+    thread_local std::string slot;
+    return slot;
+}
+}  // namespace v2g
+
 namespace {
 
-thread_local std::string g_err;
-
-struct HipError : std::runtime_error {
-    using std::runtime_error::runtime_error;
-};
-
-#define HIPCHK(expr)                                                                        \
-    do {                                                                                    \
-        hipError_t e_ = (expr);                                                             \
-        if (e_ != hipSuccess)                                                               \
-            throw HipError(std::string(#expr) + ": " + hipGetErrorString(e_));              \
-    } while (0)
-
-template <class T>
-struct DevBuf {
-    T *p = nullptr;
-    size_t n = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf &) = delete;
-    DevBuf &operator=(const DevBuf &) = delete;
-    ~DevBuf() { if (p) (void)hipFree(p); }
-    void alloc(size_t count)
-    {
-        if (p) { (void)hipFree(p); p = nullptr; }
-        n = count;
-        if (count) HIPCHK(hipMalloc(&p, sizeof(T) * count));
-    }
-    void upload(const std::vector<T> &v)
-    {
-        alloc(v.size());
-        if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice));
-    }
-};
-
-double now()
-{
-    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
+using v2g::DevBuf;
+using v2g::HipError;
+using v2g::now;
 
 constexpr int kCgChunk = 8;                 // CG iterations between host checks (even)
 constexpr int64_t kGraphMaxRows = 4 << 20;  // replay captured chunks below this many rows
@@ -182,7 +156,7 @@ struct v2g_solver {
     void mlp_scales(const double *rhs, const double *y);
     void mlp_infer(const double *rhs, const double *y);
     double mlp_train(const double *rhs, const double *rlo, int nb, int steps, double lr,
-                     std::mt19937_64 &rng);
+                     double pen, std::mt19937_64 &rng);
     // the device vectors and scalars one CG solve works in
     struct CgBufs {
         double *r, *p, *Ap, *t, *partial, *scal, *rowpart;
@@ -236,6 +210,7 @@ v2g_solver::v2g_solver(const v2g_layout_desc &d)
     dl.t1 = o.t1;
     dl.na = L->na;
     dl.nb = L->nb;
+    dl.ms = (o.multiplicity - 1.0) / 2.0;
     for (int h = 0; h < 8; h++) {
         dl.amopi[h] = o.amopi[h];
         dl.poff[h] = L->poff[h];
@@ -538,7 +513,7 @@ void v2g_solver::mlp_infer(const double *rhs, const double *y)
 // L of the base diagonal on the same probes (negative: the learned diagonal
 // is better conditioned on them).
 double v2g_solver::mlp_train(const double *rhs, const double *rlo, int nb, int steps, double lr,
-                             std::mt19937_64 &rng)
+                             double pen, std::mt19937_64 &rng)
 {
     const int64_t n = L->ncon;
     if (nb > batch_cap) {
@@ -591,7 +566,8 @@ double v2g_solver::mlp_train(const double *rhs, const double *rlo, int nb, int s
         for (auto &i : idx) i = ud(rng);
         HIPCHK(hipMemcpyAsync(bidx.p, idx.data(), sizeof(int64_t) * nb, hipMemcpyHostToDevice, st));
         v2g::mlp_batch_spec(nb, bidx.p, code.p, rownorm.p, rhs, yprev.p, rhi.p, kd.p, rlo, kd2.p,
-                            lminv.p, scales.p, feat.p, wt.p, 3.0, (double)n / nb, mlp_jacobi_base, st);
+                            lminv.p, scales.p, feat.p, wt.p, 3.0, (double)n / nb, mlp_jacobi_base, pen,
+                            st);
         v2g::mlp_grad(nb, feat.p, nullptr, wt.p, W.p, mlp_part.p, mlp_lpart.p, grad.p, dloss.p, st);
         v2g::mlp_adam(W.p, grad.p, am.p, av.p, lr, ++adam_t, st);
     }
@@ -806,13 +782,7 @@ void v2g_solver::update_xz(double mu, const double *y, const double *c, double *
 // ---------------------------------------------------------------------------
 // C ABI
 // ---------------------------------------------------------------------------
-#define API_BEGIN try {
-#define API_END                                                                           \
-    return 0;                                                                             \
-    }                                                                                     \
-    catch (const std::invalid_argument &e) { g_err = e.what(); return -1; }              \
-    catch (const HipError &e) { g_err = e.what(); return -2; }                           \
-    catch (const std::exception &e) { g_err = e.what(); return -3; }
+
 
 namespace {
 
@@ -1084,7 +1054,8 @@ int v2g_bpsdp(v2g_handle h, const v2g_params *prm, const double *c, const double
             // the base solve's final residual carries its slowest modes
             last_loss = h->mlp_train(h->Bv.p, h->r.p, prm->train_batch > 0 ? prm->train_batch : 16384,
                                      prm->train_steps > 0 ? prm->train_steps : 10,
-                                     prm->train_lr > 0 ? prm->train_lr : 2e-3, rng);
+                                     prm->train_lr > 0 ? prm->train_lr : 2e-3, prm->train_penalty,
+                                     rng);
             cg_train += iiter;
             n_train++;
             h->mlp_outer_trained++;
@@ -1100,6 +1071,8 @@ int v2g_bpsdp(v2g_handle h, const v2g_params *prm, const double *c, const double
             // converges clearly more slowly per iteration than its base did
             constexpr int kGuard = 4;
             if (iiter >= prm->cg_maxiter || !std::isfinite(res)) fallback = true;
+            // a solve twice as long as the base method's last one ends it at once
+            if (first_learned && iiter > 2 * last_base_iters + 8) fallback = true;
             if (first_learned && rate > 0.0) {
                 h->learned_rate_sum += rate;
                 if (++h->learned_rate_n == kGuard) {
@@ -1264,8 +1237,7 @@ int v2g_build_b_device(v2g_handle h, double *b)
     API_BEGIN
     if (!h || !b) throw std::invalid_argument("null argument");
     const v2g::Layout &L = *h->L;
-    const double ms = (L.opt.multiplicity - 1.0) / 2.0;
-    v2g::build_b_device(h->d_lay.p, h->d_rowsegs.p, (int)L.rowsegs.size(), L.ncon, ms, b, h->st);
+    v2g::build_b_device(h->d_lay.p, h->d_rowsegs.p, (int)L.rowsegs.size(), L.ncon, b, h->st);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(h->st));
     API_END
@@ -1517,7 +1489,7 @@ int v2g_mlp_spectral_grad(v2g_handle h, const double *rhs, const double *y, cons
     feat.alloc((size_t)n * v2g::kMlpIn);
     v2g::mlp_batch_spec((int)n, d_idx.p, h->code.p, h->rownorm.p, rhs, h->yprev.p, rhi, h->kd.p, rlo,
                         h->kd2.p, h->lminv.p, h->scales.p, feat.p, dout, 3.0, 1.0, h->mlp_jacobi_base,
-                        st);
+                        0.0, st);
     HIPCHK(hipStreamSynchronize(st));
     if (objective) *objective = std::log(ab[1] / ab[0]) - std::log(ab[3] / ab[2]);
     API_END
@@ -1545,7 +1517,7 @@ int v2g_mlp_minv(v2g_handle h, const double *rhs, const double *y, double *minv)
     API_END
 }
 
-const char *v2g_last_error(void) { return g_err.c_str(); }
+const char *v2g_last_error(void) { return v2g::last_error().c_str(); }
 const char *v2g_version(void) { return "v2g 0.1 (gfx950)"; }
 
 }  // extern "C"

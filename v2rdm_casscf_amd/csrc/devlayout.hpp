@@ -11,6 +11,7 @@ namespace v2g {
 struct DevLayout {
     int nirrep, amo, singlet, sz, spin, q2, g2, t2, t1;
     double na, nb;
+    double ms;         // M_s of the state, (multiplicity - 1) / 2
     int amopi[kMaxIrrep], poff[kMaxIrrep];
     int gems_ab[kMaxIrrep], gems_aa[kMaxIrrep];
     int trip_aab[kMaxIrrep], trip_aba[kMaxIrrep], trip_aaa[kMaxIrrep];

@@ -323,7 +323,8 @@ __device__ int pair_jacobi(double *S, double (&q)[PairShape<NT>::kRowsPerWave], 
 // ---- small blocks: whole problem in one workgroup --------------------------
 __global__ void __launch_bounds__(kT)
 k_small(const EigBlock *__restrict__ blocks, const double *__restrict__ aty,
-        const double *__restrict__ c, double *__restrict__ x, double *__restrict__ z, double mu)
+        const double *__restrict__ c, double *__restrict__ x, double *__restrict__ z, double mu0,
+        EigBatch eb)
 {
     __shared__ double S[kP * kLd];
     __shared__ double Q[kP * kLd];
@@ -331,6 +332,8 @@ k_small(const EigBlock *__restrict__ blocks, const double *__restrict__ aty,
     __shared__ double sh_c[kP / 2], sh_s[kP / 2], lam[kP], red[kT / 64];
     __shared__ int sh_pa[kP / 2], sh_pb[kP / 2], sh_cnt;
     const EigBlock B = blocks[blockIdx.x];
+    if (eb.active && !eb.active[B.prob]) return;
+    const double mu = eb.mus ? eb.mus[B.prob] : mu0;
     const int n = B.n;
     const int64_t o = B.xoff;
     double fro = 0.0;
@@ -377,11 +380,14 @@ k_small(const EigBlock *__restrict__ blocks, const double *__restrict__ aty,
 // transpose of the (P, R) one.  Cold calls also set A = M, V = I.
 __global__ void __launch_bounds__(kT)
 k_build_m(const EigBlock *__restrict__ blocks, const double *__restrict__ aty,
-          const double *__restrict__ c, const double *__restrict__ x, double mu,
-          double *__restrict__ Mw, double *__restrict__ Aw, double *__restrict__ Vw, int cold)
+          const double *__restrict__ c, const double *__restrict__ x, double mu0,
+          double *__restrict__ Mw, double *__restrict__ Aw, double *__restrict__ Vw, int cold,
+          EigBatch eb)
 {
     __shared__ double S[kP * kLd];
     const EigBlock B = blocks[blockIdx.y];
+    if (eb.active && !eb.active[B.prob]) return;
+    const double mu = eb.mus ? eb.mus[B.prob] : mu0;
     const int nt = B.N / kP;
     int64_t rem = blockIdx.x;
     if (rem >= (int64_t)nt * (nt + 1) / 2) return;
@@ -437,10 +443,11 @@ constexpr int kFroParts = 64;
 // partial squared Frobenius norms: block y, part x
 __global__ void __launch_bounds__(kT)
 k_fro_part(const EigBlock *__restrict__ blocks, const double *__restrict__ Mw,
-           double *__restrict__ part)
+           double *__restrict__ part, EigBatch eb)
 {
     __shared__ double red[kT / 64];
     const EigBlock B = blocks[blockIdx.y];
+    if (eb.active && !eb.active[B.prob]) return;
     const int64_t NN = (int64_t)B.N * B.N;
     double s = 0.0;
     for (int64_t w = blockIdx.x * (int64_t)kT + threadIdx.x; w < NN; w += (int64_t)kFroParts * kT) {
@@ -482,7 +489,7 @@ template <int NT>
 __global__ void __launch_bounds__(NT)
 k_pair_eig(const EigBlock *__restrict__ blocks, int round, const double *__restrict__ Aw,
            double *__restrict__ Qbuf, int *__restrict__ active, const double *__restrict__ tol,
-           int *__restrict__ counter, int inner_sweeps, int bip)
+           int *__restrict__ counter, int inner_sweeps, int bip, EigBatch eb)
 {
     __shared__ double S[kP * kLd];
     __shared__ double sh_c[kP / 2], sh_s[kP / 2];
@@ -493,6 +500,10 @@ k_pair_eig(const EigBlock *__restrict__ blocks, int round, const double *__restr
     const int nbk = B.N / 32;
     const int k = blockIdx.x;
     if (k >= nbk / 2 || round >= nbk - 1) return;
+    if (eb.active && !eb.active[B.prob]) {   // a problem left out: no rotation
+        if (threadIdx.x == 0) active[B.poff + k] = 0;
+        return;
+    }
     int a, b;
     circle_pair(nbk, round, k, a, b);
     const double *A = Aw + B.woff;
@@ -589,7 +600,8 @@ __device__ __forceinline__ void store_tile(double *T, const v4d (&acc)[2][2])
 // workgroups per CU, and the bound keeps the registers within that (116 VGPRs)
 __global__ void __launch_bounds__(kT, 4)
 k_apply(const EigBlock *__restrict__ blocks, int round, double *__restrict__ Aw,
-        double *__restrict__ Vw, const double *__restrict__ Qbuf, const int *__restrict__ active)
+        double *__restrict__ Vw, const double *__restrict__ Qbuf, const int *__restrict__ active,
+        EigBatch eb)
 {
     // only the tile lives in LDS; the pair rotations Q_P, Q_R are read from
     // global memory (L2-resident: each one serves a row of tiles), which
@@ -598,6 +610,7 @@ k_apply(const EigBlock *__restrict__ blocks, int round, double *__restrict__ Aw,
     const EigBlock B = blocks[blockIdx.y];
     const int nbk = B.N / 32, np = nbk / 2;
     if (round >= nbk - 1) return;
+    if (eb.active && !eb.active[B.prob]) return;
     const int64_t ntA = (int64_t)np * (np + 1) / 2, ntV = (int64_t)(B.N / kP) * np;
     const int64_t t = blockIdx.x;
     if (t >= ntA + ntV) return;
@@ -668,9 +681,11 @@ k_apply(const EigBlock *__restrict__ blocks, int round, double *__restrict__ Aw,
 // T[:, j] = V[:, j] sqrt(max(lambda_j, 0) / mu), lambda_j = A[j][j]
 __global__ void __launch_bounds__(kT)
 k_scale_vecs(const EigBlock *__restrict__ blocks, const double *__restrict__ Aw,
-             const double *__restrict__ Vw, double *__restrict__ Tw, double mu)
+             const double *__restrict__ Vw, double *__restrict__ Tw, double mu0, EigBatch eb)
 {
     const EigBlock B = blocks[blockIdx.y];
+    if (eb.active && !eb.active[B.prob]) return;
+    const double mu = eb.mus ? eb.mus[B.prob] : mu0;
     const int64_t NN = (int64_t)B.N * B.N;
     for (int64_t w = blockIdx.x * (int64_t)kT + threadIdx.x; w < NN; w += (int64_t)gridDim.x * kT) {
         const int j = (int)(w % B.N);
@@ -681,9 +696,11 @@ k_scale_vecs(const EigBlock *__restrict__ blocks, const double *__restrict__ Aw,
 
 __global__ void __launch_bounds__(kT)
 k_z(const EigBlock *__restrict__ blocks, const double *__restrict__ x,
-    const double *__restrict__ Mw, double *__restrict__ z, double mu)
+    const double *__restrict__ Mw, double *__restrict__ z, double mu0, EigBatch eb)
 {
     const EigBlock B = blocks[blockIdx.y];
+    if (eb.active && !eb.active[B.prob]) return;
+    const double mu = eb.mus ? eb.mus[B.prob] : mu0;
     const int n = B.n;
     const int64_t nn = (int64_t)n * n;
     for (int64_t w = blockIdx.x * (int64_t)kT + threadIdx.x; w < nn; w += (int64_t)gridDim.x * kT) {
@@ -695,58 +712,58 @@ k_z(const EigBlock *__restrict__ blocks, const double *__restrict__ x,
 }  // namespace
 
 void eig_small(const EigBlock *blocks, int nblocks, const double *aty, const double *c, double *x,
-               double *z, double mu, hipStream_t st)
+               double *z, double mu, hipStream_t st, EigBatch eb)
 {
     if (nblocks > 0)
-        hipLaunchKernelGGL(k_small, dim3(nblocks), dim3(kT), 0, st, blocks, aty, c, x, z, mu);
+        hipLaunchKernelGGL(k_small, dim3(nblocks), dim3(kT), 0, st, blocks, aty, c, x, z, mu, eb);
 }
 
 void eig_build_m(const EigBlock *blocks, int nblocks, int max_N, const double *aty, const double *c,
                  const double *x, double mu, double *Mw, double *Aw, double *Vw, int cold,
-                 hipStream_t st)
+                 hipStream_t st, EigBatch eb)
 {
     const int nt = max_N / kP;
     hipLaunchKernelGGL(k_build_m, dim3(nt * (nt + 1) / 2, nblocks), dim3(kT), 0, st, blocks, aty, c,
-                       x, mu, Mw, Aw, Vw, cold);
+                       x, mu, Mw, Aw, Vw, cold, eb);
 }
 
 void eig_tol(const EigBlock *blocks, int nblocks, const double *Mw, double *tol, double *part,
-             int tight, hipStream_t st)
+             int tight, hipStream_t st, EigBatch eb)
 {
-    hipLaunchKernelGGL(k_fro_part, dim3(kFroParts, nblocks), dim3(kT), 0, st, blocks, Mw, part);
+    hipLaunchKernelGGL(k_fro_part, dim3(kFroParts, nblocks), dim3(kT), 0, st, blocks, Mw, part, eb);
     hipLaunchKernelGGL(k_fro_fin, dim3((nblocks + 63) / 64), dim3(64), 0, st, blocks, nblocks, part,
                        tol, tight);
 }
 
 void eig_pairs(const EigBlock *blocks, int nblocks, int maxpairs, int round, const double *Aw,
                double *Qbuf, int *active, const double *tol, int *counter, int inner_sweeps,
-               int wide, int bip, hipStream_t st)
+               int wide, int bip, hipStream_t st, EigBatch eb)
 {
     if (wide)
         hipLaunchKernelGGL(k_pair_eig<1024>, dim3(maxpairs, nblocks), dim3(1024), 0, st, blocks, round,
-                           Aw, Qbuf, active, tol, counter, inner_sweeps, bip);
+                           Aw, Qbuf, active, tol, counter, inner_sweeps, bip, eb);
     else
         hipLaunchKernelGGL(k_pair_eig<256>, dim3(maxpairs, nblocks), dim3(256), 0, st, blocks, round,
-                           Aw, Qbuf, active, tol, counter, inner_sweeps, bip);
+                           Aw, Qbuf, active, tol, counter, inner_sweeps, bip, eb);
 }
 
 void eig_apply(const EigBlock *blocks, int nblocks, int64_t maxtiles, int round, double *Aw,
-               double *Vw, const double *Qbuf, const int *active, hipStream_t st)
+               double *Vw, const double *Qbuf, const int *active, hipStream_t st, EigBatch eb)
 {
     hipLaunchKernelGGL(k_apply, dim3((unsigned)maxtiles, nblocks), dim3(kT), 0, st, blocks, round, Aw,
-                       Vw, Qbuf, active);
+                       Vw, Qbuf, active, eb);
 }
 
 void eig_scale(const EigBlock *blocks, int nblocks, const double *Aw, const double *Vw, double *Tw,
-               double mu, hipStream_t st)
+               double mu, hipStream_t st, EigBatch eb)
 {
-    hipLaunchKernelGGL(k_scale_vecs, dim3(256, nblocks), dim3(kT), 0, st, blocks, Aw, Vw, Tw, mu);
+    hipLaunchKernelGGL(k_scale_vecs, dim3(256, nblocks), dim3(kT), 0, st, blocks, Aw, Vw, Tw, mu, eb);
 }
 
 void eig_z(const EigBlock *blocks, int nblocks, const double *x, const double *Mw, double *z,
-           double mu, hipStream_t st)
+           double mu, hipStream_t st, EigBatch eb)
 {
-    hipLaunchKernelGGL(k_z, dim3(256, nblocks), dim3(kT), 0, st, blocks, x, Mw, z, mu);
+    hipLaunchKernelGGL(k_z, dim3(256, nblocks), dim3(kT), 0, st, blocks, x, Mw, z, mu, eb);
 }
 
 }  // namespace v2g

@@ -47,11 +47,11 @@ __device__ __forceinline__ int find_start(const Segment *__restrict__ segs, int 
 // are numbered in the family's own order, as the maps number them)
 __global__ void __launch_bounds__(kT)
 k_build_b(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs, int nseg, int64_t ncon,
-          double ms, double *__restrict__ b)
+          double *__restrict__ b)
 {
-    const DevLayout &L = *Lp;
     for (int64_t r = blockIdx.x * (int64_t)kT + threadIdx.x; r < ncon; r += (int64_t)gridDim.x * kT) {
         const Segment sg = segs[find_start(segs, nseg, r)];
+        const DevLayout &L = Lp[sg.prob];
         const int64_t e = r - sg.start;
         double v = 0.0;
         switch (sg.kind) {
@@ -67,7 +67,7 @@ k_build_b(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs, in
             break;
         }
         case RF_S2:          // :2630, <S^2> target
-            v = 0.5 * (L.na + L.nb) + ms * ms - ms * (ms + 1.0);
+            v = 0.5 * (L.na + L.nb) + L.ms * L.ms - L.ms * (L.ms + 1.0);
             break;
         case RF_Q2AB:
         case RF_Q2AA: {      // :2669-2709, -delta_ik delta_jl on the diagonal
@@ -204,17 +204,17 @@ void glibc_rand_state_after_srand(unsigned seed, uint32_t (&state)[31])
     for (int i = 0; i < 31; i++) state[i] = r[344 - 31 + i];
 }
 
-void build_b_device(const DevLayout *dlay, const Segment *rowsegs, int nseg, int64_t ncon, double ms,
-                    double *b, hipStream_t st)
+void build_b_device(const DevLayout *dlay, const Segment *rowsegs, int nseg, int64_t ncon, double *b,
+                    hipStream_t st)
 {
     if (ncon <= 0) return;
-    hipLaunchKernelGGL(k_build_b, dim3(2048), dim3(kT), 0, st, dlay, rowsegs, nseg, ncon, ms, b);
+    hipLaunchKernelGGL(k_build_b, dim3(2048), dim3(kT), 0, st, dlay, rowsegs, nseg, ncon, b);
 }
 
 void build_c_device(const DevLayout *dlay, int nirrep, int max_gab, int max_gaa, int max_m,
                     int64_t dimx, const double *h1, const double *eri, double *c, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_fill, dim3(2048), dim3(kT), 0, st, dimx, 0.0, c);
+    if (dimx > 0) hipLaunchKernelGGL(k_fill, dim3(2048), dim3(kT), 0, st, dimx, 0.0, c);
     auto grid = [](int64_t n) { return (unsigned)std::max<int64_t>(1, std::min<int64_t>(4096, (n + kT - 1) / kT)); };
     if (max_gab > 0)
         hipLaunchKernelGGL(k_build_c_d2, dim3(grid((int64_t)max_gab * max_gab), nirrep), dim3(kT), 0, st,

@@ -13,9 +13,9 @@ namespace v2g {
 // outputs of the glibc generator produced by one thread of the guess kernel
 constexpr int64_t kGuessChunk = 65536;
 
-// BuildConstraints: b over all ncon rows (ms = M_s of the state)
-void build_b_device(const DevLayout *dlay, const Segment *rowsegs, int nseg, int64_t ncon, double ms,
-                    double *b, hipStream_t st);
+// BuildConstraints: b over all ncon rows (of every problem of a batch plan)
+void build_b_device(const DevLayout *dlay, const Segment *rowsegs, int nseg, int64_t ncon, double *b,
+                    hipStream_t st);
 // RepackIntegrals without core orbitals: c over dimx entries
 void build_c_device(const DevLayout *dlay, int nirrep, int max_gab, int max_gaa, int max_m,
                     int64_t dimx, const double *h1, const double *eri, double *c, hipStream_t st);

@@ -28,6 +28,9 @@ struct MapPlan {
     // precomputed (D2 column << 1 | sign, D1 column) of the G2 rows, indexed
     // by each G2 row segment's aux + visiting position (null: not built)
     const int2 *g2idx = nullptr;
+    // dlay holds one layout per problem (Segment::prob picks it); the done flag
+    // of problem p is done[p * done_stride] (0: one flag for the whole plan)
+    int done_stride = 0;
 };
 
 int64_t tiles_for(const Segment &s);

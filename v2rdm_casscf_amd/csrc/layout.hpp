@@ -69,7 +69,7 @@ struct Segment {
     int64_t aux;       // XS_SELF: first source row; else unused
     double coef;       // XS_SELF: coefficient
     int32_t wave;      // 1: one wavefront per item
-    int32_t pad;
+    int32_t prob;      // problem of a batch whose layout the segment follows (0 alone)
 };
 
 struct Options {

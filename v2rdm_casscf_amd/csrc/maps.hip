@@ -125,8 +125,8 @@ __global__ void __launch_bounds__(kBlock)
 k_g2_index(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs, int si, int64_t d1lo,
            int64_t d1hi, int2 *__restrict__ idx)
 {
-    const DevLayout &L = *Lp;
     const Segment sg = segs[si];
+    const DevLayout &L = Lp[sg.prob];
     for (int64_t pos = blockIdx.x * (int64_t)kBlock + threadIdx.x; pos < sg.count;
          pos += (int64_t)gridDim.x * kBlock) {
         G2Rec f{d1lo, d1hi};
@@ -139,14 +139,15 @@ __global__ void __launch_bounds__(kBlock)
 k_au(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
      const int64_t *__restrict__ tile_start, int nseg, const double *__restrict__ x,
      double *__restrict__ Ax, const double *__restrict__ done, const double *__restrict__ dotv,
-     double *__restrict__ rowpart, const int2 *__restrict__ g2idx)
+     double *__restrict__ rowpart, const int2 *__restrict__ g2idx, int done_stride)
 {
-    if (done && *done != 0.0) return;
     __shared__ double red[kWavesPerBlock];
-    const DevLayout &L = *Lp;
     const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const int si = find_segment(tile_start, nseg, tile);
     const Segment sg = segs[si];
+    // done flag of this segment's problem (a batch keeps one per problem)
+    if (done && done[(int64_t)sg.prob * done_stride] != 0.0) return;
+    const DevLayout &L = Lp[sg.prob];
     const int64_t t = tile - tile_start[si];
     double contrib = 0.0;
     if (sg.wave) {
@@ -237,10 +238,10 @@ __global__ void __launch_bounds__(kBlock)
 k_row_norm2(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
             const int64_t *__restrict__ tile_start, int nseg, double *__restrict__ d)
 {
-    const DevLayout &L = *Lp;
     const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
     const int si = find_segment(tile_start, nseg, tile);
     const Segment sg = segs[si];
+    const DevLayout &L = Lp[sg.prob];
     const int64_t t = tile - tile_start[si];
     if (sg.wave) {
         const int64_t item = t * kWavesPerBlock + (threadIdx.x >> 6);
@@ -276,10 +277,8 @@ k_row_norm2(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
 __global__ void __launch_bounds__(kBlock, 8)
 k_atu(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
       const int64_t *__restrict__ tile_start, int nseg, const double *__restrict__ y,
-      double *__restrict__ ATy, const double *__restrict__ done, int64_t ntiles)
+      double *__restrict__ ATy, const double *__restrict__ done, int64_t ntiles, int done_stride)
 {
-    if (done && *done != 0.0) return;
-    const DevLayout &L = *Lp;
     // chunked (not contiguous) XCD ranges: the primal segments differ widely in
     // cost (closed-form D2 gathers vs copies), and contiguous per-XCD ranges
     // would leave some XCDs with all of the expensive tiles
@@ -287,6 +286,8 @@ k_atu(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
     if (tile >= ntiles) return;
     const int si = find_segment(tile_start, nseg, tile);
     const Segment sg = segs[si];
+    if (done && done[(int64_t)sg.prob * done_stride] != 0.0) return;
+    const DevLayout &L = Lp[sg.prob];
     const int64_t t = tile - tile_start[si];
     if (sg.wave) {
         const int64_t item = t * kWavesPerBlock + (threadIdx.x >> 6);
@@ -320,13 +321,13 @@ k_atu(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
 __global__ void __launch_bounds__(kBlock)
 k_atu_t2(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
          const int64_t *__restrict__ tile_start, int nseg, const double *__restrict__ y,
-         double *__restrict__ ATy, const double *__restrict__ done)
+         double *__restrict__ ATy, const double *__restrict__ done, int done_stride)
 {
-    if (done && *done != 0.0) return;
-    const DevLayout &L = *Lp;
     const int64_t tile = blockIdx.x;
     const int si = find_segment(tile_start, nseg, tile);
     const Segment sg = segs[si];
+    if (done && done[(int64_t)sg.prob * done_stride] != 0.0) return;
+    const DevLayout &L = Lp[sg.prob];
     for (int u = 0; u < kPerThread; u++) {
         const int64_t e = (tile - tile_start[si]) * kElemTile + u * kBlock + threadIdx.x;
         if (e >= sg.count) return;
@@ -345,7 +346,7 @@ void launch_au(const MapPlan &p, const double *x, double *Ax, hipStream_t st, co
     if (p.row_tiles > 0)
         hipLaunchKernelGGL(k_au, dim3((unsigned)p.row_tiles), dim3(kBlock), 0, st, p.dlay,
                            p.rowsegs, p.row_tile_start, p.nrowsegs, x, Ax, done, dotv, rowpart,
-                           p.g2idx);
+                           p.g2idx, p.done_stride);
 }
 
 void build_g2_index(const MapPlan &p, const std::vector<Segment> &rowsegs, int64_t d1lo,
@@ -371,10 +372,10 @@ void launch_atu(const MapPlan &p, const double *y, double *ATy, hipStream_t st, 
 {
     if (p.x_tiles > 0)
         hipLaunchKernelGGL(k_atu, dim3((unsigned)chunk_grid(p.x_tiles)), dim3(kBlock), 0, st, p.dlay,
-                           p.xsegs, p.x_tile_start, p.nxsegs, y, ATy, done, p.x_tiles);
+                           p.xsegs, p.x_tile_start, p.nxsegs, y, ATy, done, p.x_tiles, p.done_stride);
     if (p.t2_tiles > 0)
         hipLaunchKernelGGL(k_atu_t2, dim3((unsigned)p.t2_tiles), dim3(kBlock), 0, st, p.dlay,
-                           p.t2segs, p.t2_tile_start, p.nt2segs, y, ATy, done);
+                           p.t2segs, p.t2_tile_start, p.nt2segs, y, ATy, done, p.done_stride);
 }
 
 int64_t tiles_for(const Segment &s)

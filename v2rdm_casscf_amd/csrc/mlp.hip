@@ -113,8 +113,9 @@ k_mlp_minv(int64_t n, const unsigned char *__restrict__ code, const double *__re
 // so L estimates the log condition number.  The upstream gradient with
 // respect to g_i (s_i = exp(g_i) / base_i) is
 //   dL/dg_i = [2 s_i r_i (K S r)_i / b - s_i r_i^2 / a]_hi - [ ... ]_lo
-// with a = r.Sr and b = (Sr).K(Sr) (scales[2..5] = a_hi, b_hi, a_lo, b_lo);
-// zero where g is clamped.  `coef` = ncon / nb.
+// with a = r.Sr and b = (Sr).K(Sr) (scales[2..5] = a_hi, b_hi, a_lo, b_lo),
+// plus pen g_i / ncon from the penalty (pen / 2) mean(g^2); zero where g is
+// clamped.  `coef` = ncon / nb.
 __global__ void __launch_bounds__(kT)
 k_mlp_batch_spec(int nb, const int64_t *__restrict__ idx, const unsigned char *__restrict__ code,
                  const double *__restrict__ d, const double *__restrict__ rhs,
@@ -122,7 +123,8 @@ k_mlp_batch_spec(int nb, const int64_t *__restrict__ idx, const unsigned char *_
                  const double *__restrict__ kdhi, const double *__restrict__ rlo,
                  const double *__restrict__ kdlo, const double *__restrict__ s,
                  const double *__restrict__ scales, float *__restrict__ feat,
-                 float *__restrict__ dout, double clampv, double coef, int jacobi_base)
+                 float *__restrict__ dout, double clampv, double coef, int jacobi_base,
+                 double pen)
 {
     const int b = blockIdx.x * kT + threadIdx.x;
     if (b >= nb) return;
@@ -137,7 +139,8 @@ k_mlp_batch_spec(int nb, const int64_t *__restrict__ idx, const unsigned char *_
     if (fabs(g) < clampv - 1e-6 && ahi > 0.0 && bhi > 0.0 && alo > 0.0 && blo > 0.0) {
         const double uh = 2.0 * s[i] * rhi[i] * kdhi[i] / bhi - s[i] * rhi[i] * rhi[i] / ahi;
         const double ul = 2.0 * s[i] * rlo[i] * kdlo[i] / blo - s[i] * rlo[i] * rlo[i] / alo;
-        u = uh - ul;
+        // + d/dg_i of (pen / 2) mean(g^2): pulls the diagonal towards its base
+        u = uh - ul + pen * g / (coef * nb);
     }
     dout[b] = (float)(coef * u);
 }
@@ -330,11 +333,11 @@ void mlp_batch_spec(int nb, const int64_t *idx, const unsigned char *code, const
                     const double *rhs, const double *yprev, const double *rhi, const double *kdhi,
                     const double *rlo, const double *kdlo, const double *s, const double *scales,
                     float *feat, float *dout, double clampv, double coef, int jacobi_base,
-                    hipStream_t st)
+                    double pen, hipStream_t st)
 {
     hipLaunchKernelGGL(k_mlp_batch_spec, dim3((nb + kT - 1) / kT), dim3(kT), 0, st, nb, idx, code, d,
                        rhs, yprev, rhi, kdhi, rlo, kdlo, s, scales, feat, dout, clampv, coef,
-                       jacobi_base);
+                       jacobi_base, pen);
 }
 
 int mlp_grad_blocks(int nb) { return (nb + kS - 1) / kS; }

@@ -28,7 +28,7 @@ void mlp_batch_spec(int nb, const int64_t *idx, const unsigned char *code, const
                     const double *rhs, const double *yprev, const double *rhi, const double *kdhi,
                     const double *rlo, const double *kdlo, const double *s, const double *scales,
                     float *feat, float *dout, double clampv, double coef, int jacobi_base,
-                    hipStream_t st);
+                    double pen, hipStream_t st);
 int mlp_grad_blocks(int nb);
 // tgt == nullptr: wt holds the upstream gradient d(loss)/d(out) per sample
 void mlp_grad(int nb, const float *feat, const float *tgt, const float *wt, const float *W,

@@ -81,7 +81,8 @@ class V2RDMSolver:
                             MU_UPDATE_FREQUENCY=500, PRECONDITIONER="NONE", FLEXIBLE_CG=False,
                             MLP_TRAIN_ITERS=30, MLP_TRAIN_BATCH=16384, MLP_TRAIN_STEPS=10,
                             MLP_TRAIN_LR=2e-3, EIG_WARM_SWEEPS=1, EIG_INNER_SWEEPS=1,
-                            CG_EXTRAPOLATE=True, MLP_BASE="IDENTITY", EIG_PAIR_ORDER=1)
+                            CG_EXTRAPOLATE=True, MLP_BASE="IDENTITY", EIG_PAIR_ORDER=1,
+                            MLP_TRAIN_PENALTY=0.0)
         if options:
             self.options.update({k.upper(): v for k, v in options.items()})
         self.x = self.z = self.y = self.c = self.b = None
@@ -381,6 +382,7 @@ class V2RDMSolver:
                         int(o["EIG_INNER_SWEEPS"]), int(bool(o["CG_EXTRAPOLATE"])))
         p.mlp_base = {"IDENTITY": 0, "JACOBI": 1}[str(o["MLP_BASE"]).upper()]
         p.eig_pair_order = int(o["EIG_PAIR_ORDER"])
+        p.train_penalty = float(o["MLP_TRAIN_PENALTY"])
         prev = self.last_stats if resume else None
         if prev:
             p.mu0, p.oiter0, p.ep0, p.ed0 = prev["mu"], prev["oiter"], prev["ep"], prev["ed"]
