@@ -129,7 +129,9 @@ typedef struct v2g_params {
     int mu_update_frequency;     /* MU_UPDATE_FREQUENCY                   */
     int precond;                 /* as v2g_cg_solve, or 3: learned online  */
     int flexible;
-    int verbose;                 /* 1: one line per outer iteration       */
+    int verbose;                 /* 1: one line per outer iteration;      */
+                                 /* 2 (batches): a progress line on       */
+                                 /* stderr every 20 s                     */
     double max_seconds;          /* > 0: stop after this wall time        */
     int train_iters;             /* precond 3: outer iterations of data   */
     int train_batch;             /* precond 3: rows sampled per iteration */

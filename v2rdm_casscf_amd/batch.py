@@ -102,7 +102,7 @@ class BatchSolver:
         check(self.lib.v2g_batch_inputs(self.h, ph, pe, _ptr(self.c), _ptr(self.b), _ptr(self.x),
                                         _ptr(self.z), _ptr(self.y)))
 
-    def compute_energy(self, enucs=None, max_seconds=0.0):
+    def compute_energy(self, enucs=None, max_seconds=0.0, progress=False):
         """The BPSDP loop of every problem; returns a list of stats dicts."""
         if self.c is None:
             raise ValueError("no integrals: call set_integrals first")
@@ -112,6 +112,7 @@ class BatchSolver:
         p.cg_convergence, p.maxiter, p.cg_maxiter = o["CG_CONVERGENCE"], int(o["MAXITER"]), int(o["CG_MAXITER"])
         p.mu_update_frequency, p.precond = int(o["MU_UPDATE_FREQUENCY"]), 0
         p.max_seconds = float(max_seconds)
+        p.verbose = 2 if progress else 0     # a progress line on stderr every 20 s
         p.eig_warm_sweeps, p.eig_inner_sweeps = int(o["EIG_WARM_SWEEPS"]), int(o["EIG_INNER_SWEEPS"])
         p.cg_extrapolate, p.eig_pair_order = int(bool(o["CG_EXTRAPOLATE"])), int(o["EIG_PAIR_ORDER"])
         st = (_lib.Stats * self.nprob)()

@@ -47,7 +47,8 @@ def solve_scan(points, positivity="DQG", device="cuda:0", options=None, max_seco
     if progress:
         progress(-1, dict(stage=f"{len(points)} problems in one batch"))
     t0 = time.perf_counter()
-    out = bs.compute_energy(enucs=[d["enuc"] for d in hams], max_seconds=max_seconds)
+    out = bs.compute_energy(enucs=[d["enuc"] for d in hams], max_seconds=max_seconds,
+                            progress=progress is not None)
     wall = time.perf_counter() - t0
     for i, (r, d) in enumerate(zip(out, hams)):
         r.update(n=points[i][0], spacing_angstrom=points[i][1], e_rhf=d["e_rhf"])

@@ -26,6 +26,7 @@
 #include <climits>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <vector>
@@ -352,12 +353,9 @@ struct v2g_batch {
         if (cg_exec) (void)hipGraphExecDestroy(cg_exec);
         if (st) (void)hipStreamDestroy(st);
     }
-    dim3 grid_of(int space) const
-    {
-        int64_t mx = 1;
-        for (const auto &q : pr) mx = std::max(mx, space ? q.nx : q.nr);
-        return dim3((unsigned)std::min<int64_t>(v2g::kBRed, (mx + v2g::kT - 1) / v2g::kT), (unsigned)np);
-    }
+    // kBRed blocks per problem, always: the reductions sum exactly that many
+    // partials (blocks past a small problem's end contribute zeros)
+    dim3 grid_of(int) const { return dim3((unsigned)v2g::kBRed, (unsigned)np); }
     void seg_axpby(int space, const int *act, const std::vector<double> &a, const double *u,
                    const std::vector<double> &b, const double *v, double *out);
     std::vector<double> seg_dot(int space, const double *u, const double *v);
@@ -887,7 +885,19 @@ int v2g_batch_bpsdp(v2g_batch_handle h, const v2g_params *prm, const double *c, 
                  std::fabs(ecur[q] - edual[q]) > prm->e_convergence);
     };
     int nact = np;
+    double t_report = t0;
+    int bit = 0;
     while (nact > 0) {
+        // verbose 2: one progress line every 20 s (long batched solves)
+        if (prm->verbose == 2 && v2g::now() - t_report > 20.0) {
+            t_report = v2g::now();
+            int mx = 0;
+            for (int q = 0; q < np; q++) mx = std::max(mx, oiter[q]);
+            std::fprintf(stderr, "# batch: %d of %d problems still iterating, %d iterations, "
+                         "largest count %d, %.1f s\n", nact, np, bit, mx, t_report - t0);
+            std::fflush(stderr);
+        }
+        bit++;
         const double t1 = v2g::now();
         upload_act(act);
         // Ax = -tau mu (A x - b); z = c - z; B = A z + Ax  (v2rdm_solver.cc:1784-1794)
