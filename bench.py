@@ -244,6 +244,40 @@ def _scan(args):
                                           converged=r["converged"]) for r in res])))
 
 
+PC_STUDY_SETTINGS = [(2e-3, 10, 0.0), (5e-4, 10, 0.0), (2e-3, 10, 1.0), (2e-3, 10, 10.0),
+                     (5e-4, 3, 1.0), (2e-3, 10, 100.0)]
+
+
+def _pc_study(args):
+    """Learned preconditioner against plain CG to convergence on H8 and H12
+    (STO-3G, 1.8 bohr, DQG): one JSON line per run, plain CG first, then the
+    learned diagonal for each (learning rate, Adam steps per outer iteration,
+    penalty) of PC_STUDY_SETTINGS, trained over the first --train-iters outer
+    iterations."""
+    from v2rdm_casscf_amd import V2RDMSolver
+    from v2rdm_casscf_amd.hamiltonians import hydrogen_chain
+    for n in (8, 12):
+        d = hydrogen_chain(n, 1.8)
+
+        def run(pc, **kw):
+            o = dict(PRECONDITIONER=pc, MLP_TRAIN_ITERS=args.train_iters, MAXITER=100000,
+                     EIG_WARM_SWEEPS=args.eig_sweeps, EIG_INNER_SWEEPS=args.inner_sweeps,
+                     EIG_PAIR_ORDER=args.eig_order)
+            o.update(kw)
+            s = V2RDMSolver(d["amopi"], n // 2, n // 2, 1, "DQG", options=o)
+            t0 = time.perf_counter()
+            r = s.compute_energy(d["h1"], d["eri"], enuc=d["enuc"])
+            return dict(n=n, pc=pc, **kw, oiter=r["oiter"], cg=r["cg_iters"],
+                        cg_per_outer=round(r["cg_iters"] / r["oiter"], 3),
+                        fallback=r["fallback"], seconds=round(time.perf_counter() - t0, 2),
+                        objective=r["mlp_loss"], energy=r["energy"])
+
+        print(json.dumps(run("NONE")), flush=True)
+        for lr, steps, pen in PC_STUDY_SETTINGS:
+            print(json.dumps(run("LEARNED", MLP_TRAIN_LR=lr, MLP_TRAIN_STEPS=steps,
+                                 MLP_TRAIN_PENALTY=pen)), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -286,6 +320,9 @@ def main():
     ap.add_argument("--maps-only", action="store_true",
                     help="only the A^T.y / A.x kernels (map-reps calls each) and a device copy "
                          "of 8 (dimx + ncon) bytes as the traffic reference, for counter runs")
+    ap.add_argument("--pc-study", action="store_true",
+                    help="learned preconditioner vs plain CG to convergence on H8 and H12 over "
+                         "a sweep of training settings (one JSON line per run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one MI355X (--gpus 1); --gpus {args.gpus} is not supported: "
@@ -297,6 +334,9 @@ def main():
     from v2rdm_casscf_amd import V2RDMSolver
     if args.scan > 0:
         _scan(args)
+        return
+    if args.pc_study:
+        _pc_study(args)
         return
 
     dev = torch.device("cuda:0")

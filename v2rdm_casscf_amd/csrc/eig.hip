@@ -485,6 +485,16 @@ __device__ __forceinline__ int pair_index(int a, int b, int i)
     return i < 32 ? a * 32 + i : b * 32 + (i - 32);
 }
 
+// the column blocks of pair k in round r, lower block first: the rotated
+// matrix A is kept as its upper triangle of 32 x 32 sub-blocks (sub-block
+// (u, v) with u > v is stale; it is the transpose of (v, u)), and with a < b
+// the off-diagonal sub-block of every pair problem is a stored one
+__device__ __forceinline__ void ordered_pair(int m, int r, int k, int &a, int &b)
+{
+    circle_pair(m, r, k, a, b);
+    if (a > b) { const int t = a; a = b; b = t; }
+}
+
 template <int NT>
 __global__ void __launch_bounds__(NT)
 k_pair_eig(const EigBlock *__restrict__ blocks, int round, const double *__restrict__ Aw,
@@ -505,7 +515,7 @@ k_pair_eig(const EigBlock *__restrict__ blocks, int round, const double *__restr
         return;
     }
     int a, b;
-    circle_pair(nbk, round, k, a, b);
+    ordered_pair(nbk, round, k, a, b);
     const double *A = Aw + B.woff;
     const double tl = tol[blockIdx.y];
     if (threadIdx.x == 0) sh_cnt = 0;
@@ -519,17 +529,21 @@ k_pair_eig(const EigBlock *__restrict__ blocks, int round, const double *__restr
     {
         // thread (i0, j) loads rows i0, i0 + NT / kP, ... of column j, all
         // loads in flight at once
+        // the upper triangle of the pair matrix is all pair_jacobi reads: its
+        // (b, a) sub-block is stale in A and is not loaded
         const int j = threadIdx.x & (kP - 1), i0 = threadIdx.x / kP;
         const int64_t colj = pair_index(a, b, j);
         double v[kP * kP / NT];
 #pragma unroll
-        for (int u = 0; u < kP * kP / NT; u++)
-            v[u] = A[(int64_t)pair_index(a, b, i0 + u * (NT / kP)) * B.N + colj];
+        for (int u = 0; u < kP * kP / NT; u++) {
+            const int i = i0 + u * (NT / kP);
+            v[u] = (i >= 32 && j < 32) ? 0.0 : A[(int64_t)pair_index(a, b, i) * B.N + colj];
+        }
 #pragma unroll
         for (int u = 0; u < kP * kP / NT; u++) {
             const int i = i0 + u * (NT / kP);
             S[i * kLd + j] = v[u];
-            big |= (i != j && fabs(v[u]) > tl);
+            big |= (i < j && fabs(v[u]) > tl);
         }
     }
     if (big) atomicOr(&sh_cnt, 1);
@@ -595,8 +609,8 @@ __device__ __forceinline__ void store_tile(double *T, const v4d (&acc)[2][2])
                 T[(wr + a * 16 + 4 * r + (lane >> 4)) * kLd + wc + b * 16 + (lane & 15)] = acc[a][b][r];
 }
 
-// A_PR <- Q_P^T A_PR Q_R for the pair tiles with P <= R (A is symmetric: the
-// (R, P) tile is written as the transpose); V[:, R] <- V[:, R] Q_R.  Four waves per SIMD: the LDS tile allows four
+// A_PR <- Q_P^T A_PR Q_R for the pair tiles with P <= R (A is symmetric and
+// kept as its upper triangle of 32 x 32 sub-blocks); V[:, R] <- V[:, R] Q_R.  Four waves per SIMD: the LDS tile allows four
 // workgroups per CU, and the bound keeps the registers within that (116 VGPRs)
 __global__ void __launch_bounds__(kT, 4)
 k_apply(const EigBlock *__restrict__ blocks, int round, double *__restrict__ Aw,
@@ -630,18 +644,54 @@ k_apply(const EigBlock *__restrict__ blocks, int round, double *__restrict__ Aw,
     const bool actR = active[B.poff + R];
     if (!actP && !actR) return;
     int pa = 0, pb = 0, ra, rb;
-    if (isA) circle_pair(nbk, round, P, pa, pb);
-    circle_pair(nbk, round, R, ra, rb);
+    if (isA) ordered_pair(nbk, round, P, pa, pb);
+    ordered_pair(nbk, round, R, ra, rb);
     double *M = isA ? Aw + B.woff : Vw + B.woff;
-    auto row_of = [&](int i) -> int64_t { return isA ? pair_index(pa, pb, i) : rt * kP + i; };
-    // thread (i0, j) moves rows i0, i0 + 4, ... of column j: all 16 loads are
-    // issued before the first LDS write, so each thread keeps 16 in flight
     const int j = threadIdx.x & (kP - 1), i0 = threadIdx.x / kP;
     const int64_t colj = pair_index(ra, rb, j);
-    {
+    // A tiles move as four 32 x 32 sub-blocks (u, v), u in {pa, pb}, v in
+    // {ra, rb}: a stored one (u <= v) directly, a stale one (u > v) as the
+    // transpose of (v, u); thread t moves stored rows t / 32 + 8 k, column
+    // t % 32 of each (coalesced 256-byte rows either way).  Each unordered
+    // pair of column blocks belongs to one tile of the round, so every
+    // stored sub-block is written once; in a diagonal tile (P == R) the
+    // (pb, pa) sub-block duplicates (pa, pb) and is not written back.
+    const int sr0 = threadIdx.x >> 5, sc = threadIdx.x & 31;
+    auto sub_addr = [&](int ub, int vb, int k, int &ti, int &tj) -> int64_t {
+        const int u = ub ? pb : pa, v = vb ? rb : ra, sr = sr0 + 8 * k;
+        if (u <= v) {
+            ti = ub * 32 + sr;
+            tj = vb * 32 + sc;
+            return (int64_t)(u * 32 + sr) * B.N + v * 32 + sc;
+        }
+        ti = ub * 32 + sc;
+        tj = vb * 32 + sr;
+        return (int64_t)(v * 32 + sr) * B.N + u * 32 + sc;
+    };
+    if (isA) {
+        double v[16];
+#pragma unroll
+        for (int s = 0; s < 4; s++)
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                int ti, tj;
+                v[s * 4 + k] = M[sub_addr(s >> 1, s & 1, k, ti, tj)];
+            }
+#pragma unroll
+        for (int s = 0; s < 4; s++)
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                int ti, tj;
+                (void)sub_addr(s >> 1, s & 1, k, ti, tj);
+                T[ti * kLd + tj] = v[s * 4 + k];
+            }
+    } else {
+        // V tiles: thread (i0, j) moves rows i0, i0 + 4, ... of column j, all
+        // 16 loads issued before the first LDS write
         double v[kP * kP / kT];
 #pragma unroll
-        for (int u = 0; u < kP * kP / kT; u++) v[u] = M[row_of(i0 + u * (kT / kP)) * B.N + colj];
+        for (int u = 0; u < kP * kP / kT; u++)
+            v[u] = M[(int64_t)(rt * kP + i0 + u * (kT / kP)) * B.N + colj];
 #pragma unroll
         for (int u = 0; u < kP * kP / kT; u++) T[(i0 + u * (kT / kP)) * kLd + j] = v[u];
     }
@@ -661,19 +711,22 @@ k_apply(const EigBlock *__restrict__ blocks, int round, double *__restrict__ Aw,
         store_tile(T, acc);
     }
     __syncthreads();
+    if (isA) {
 #pragma unroll
-    for (int u = 0; u < kP * kP / kT; u++) {
-        const int i = i0 + u * (kT / kP);
-        M[row_of(i) * B.N + colj] = T[i * kLd + j];
-    }
-    if (isA && P != R) {
-        // (R, P) tile = transpose: thread (i0, j) writes row pair_index(ra, rb, i)
-        // of the (R, P) tile at column pair_index(pa, pb, j)
-        const int64_t colt = pair_index(pa, pb, j);
+        for (int s = 0; s < 4; s++) {
+            if (P == R && s == 2) continue;   // (pb, pa) of a diagonal tile
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                int ti, tj;
+                const int64_t a = sub_addr(s >> 1, s & 1, k, ti, tj);
+                M[a] = T[ti * kLd + tj];
+            }
+        }
+    } else {
 #pragma unroll
         for (int u = 0; u < kP * kP / kT; u++) {
             const int i = i0 + u * (kT / kP);
-            M[(int64_t)pair_index(ra, rb, i) * B.N + colt] = T[j * kLd + i];
+            M[(int64_t)(rt * kP + i) * B.N + colj] = T[i * kLd + j];
         }
     }
 }
