@@ -31,6 +31,16 @@ typedef struct v2g_layout_desc {
     int constrain_q2, constrain_g2, constrain_t2;
     int constrain_spin, constrain_sz;
     int constrain_t1;      /* POSITIVITY DQGT1 / DQGT1T2 / DQGT              */
+    int spin_adapt_q2;     /* SPIN_ADAPT_Q2 (plugin.cc:110): Q2s, Q2t, Q2t(+1),*/
+                           /* Q2t(-1) blocks (q2.cc:260-670)                 */
+    int spin_adapt_g2;     /* SPIN_ADAPT_G2 (plugin.cc:108): G2s, G2t, G2t(+1),*/
+                           /* G2t(-1) blocks (g2.cc:235-856); needs           */
+                           /* constrain_spin = 0, whose maximal-spin rows     */
+                           /* address G2ba (spin.cc:250-268)                  */
+    const int *energy_order; /* active orbitals from lowest to highest       */
+                           /* orbital energy: the order of the singlet       */
+                           /* geminals of spin-adapted Q2 (basis.cc:626-651);*/
+                           /* NULL = Pitzer order                            */
 } v2g_layout_desc;
 
 /* Replaces v2RDMSolver construction + common_init sizes (v2rdm_solver.cc:

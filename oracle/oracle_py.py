@@ -13,7 +13,8 @@ LIB_PATH = os.path.join(_HERE, "build", "liboracle_v2rdm.so")
 
 OFF = dict(D2AB=0, D2AA=1, D2BB=2, D200=3, D1A=4, D1B=5, Q1A=6, Q1B=7, Q2AB=8, Q2AA=9,
            Q2BB=10, G2AB=11, G2BA=12, G2AA=13, T2AAA=14, T2BBB=15, T2AAB=16, T2BBA=17,
-           T1AAA=18, T1BBB=19, T1AAB=20, T1BBA=21)
+           T1AAA=18, T1BBB=19, T1AAB=20, T1BBA=21, Q2S=22, Q2T=23, Q2TP1=24, Q2TM1=25,
+           G2S=26, G2T=27, G2TP1=28, G2TM1=29)
 
 
 class _Options(ctypes.Structure):
@@ -22,7 +23,8 @@ class _Options(ctypes.Structure):
                 ("multiplicity", ctypes.c_int), ("constrain_q2", ctypes.c_int),
                 ("constrain_g2", ctypes.c_int), ("constrain_t2", ctypes.c_int),
                 ("constrain_spin", ctypes.c_int), ("constrain_sz", ctypes.c_int),
-                ("constrain_t1", ctypes.c_int)]
+                ("constrain_t1", ctypes.c_int), ("spin_adapt_q2", ctypes.c_int),
+                ("spin_adapt_g2", ctypes.c_int), ("energy_order", ctypes.POINTER(ctypes.c_int))]
 
 
 def positivity_flags(positivity):
@@ -73,12 +75,13 @@ def lib():
         L.orc_offset.restype = ctypes.c_long
         L.orc_offset.argtypes = [vp, ctypes.c_int, ctypes.c_int]
         for f in ("orc_gems_ab", "orc_gems_aa", "orc_trip_aab", "orc_trip_aba", "orc_trip_aaa",
-                  "orc_orbital_irrep"):
+                  "orc_orbital_irrep", "orc_gems_00"):
             getattr(L, f).restype = ctypes.c_int
             getattr(L, f).argtypes = [vp, ctypes.c_int]
         for f in ("orc_ibas_ab", "orc_ibas_aa"):
             getattr(L, f).restype = ctypes.c_int
             getattr(L, f).argtypes = [vp, ctypes.c_int, ctypes.c_int]
+        L.orc_bas_00.argtypes = [vp, ctypes.c_int, ip]
         L.orc_row_families.restype = ctypes.c_int
         L.orc_row_families.argtypes = [vp, ctypes.c_int, ctypes.c_char_p,
                                        ctypes.POINTER(ctypes.c_long),
@@ -119,7 +122,8 @@ class Layout:
     """Oracle layout: geminal tables, x offsets, constraint rows."""
 
     def __init__(self, amopi, nalpha, nbeta, multiplicity=None, positivity="DQG",
-                 constrain_spin=True, constrain_sz=True):
+                 constrain_spin=True, constrain_sz=True, spin_adapt_q2=False,
+                 spin_adapt_g2=False, energy_order=None):
         L = lib()
         amopi = list(amopi)
         nirrep = len(amopi)
@@ -133,6 +137,23 @@ class Layout:
         o.constrain_q2, o.constrain_g2, o.constrain_t1, o.constrain_t2 = positivity_flags(p)
         o.constrain_spin = int(constrain_spin)
         o.constrain_sz = int(constrain_sz)
+        o.spin_adapt_q2 = int(bool(spin_adapt_q2))
+        o.spin_adapt_g2 = int(bool(spin_adapt_g2))
+        amo = sum(amopi)
+        self._eorder = None
+        if energy_order is not None:
+            eo = [int(v) for v in energy_order]
+            if sorted(eo) != list(range(amo)):
+                raise ValueError("energy_order must be a permutation of the active orbitals")
+            self._eorder = (ctypes.c_int * amo)(*eo)
+            o.energy_order = ctypes.cast(self._eorder, ctypes.POINTER(ctypes.c_int))
+        if (spin_adapt_q2 or spin_adapt_g2) and (o.constrain_t1 or o.constrain_t2):
+            raise ValueError("If constraining T1/T2, G2/Q2 cannot currently be spin adapted.")
+        if (spin_adapt_q2 or spin_adapt_g2) and o.multiplicity > 1:
+            raise ValueError("Q2 / G2 not spin adapted for S = M != 0")
+        if spin_adapt_g2 and o.constrain_g2 and constrain_spin:
+            raise ValueError("SPIN_ADAPT_G2 needs CONSTRAIN_SPIN false: the maximal-spin rows "
+                             "address the G2ba block, which spin adaptation replaces")
         self.opts = o
         self.amopi = amopi
         self.nirrep = nirrep
@@ -162,6 +183,16 @@ class Layout:
 
     def gems_aa(self, h):
         return lib().orc_gems_aa(self.h, h)
+
+    def gems_00(self, h):
+        return lib().orc_gems_00(self.h, h)
+
+    def bas_00(self, h):
+        """singlet geminals (i, j) of irrep h in their order"""
+        n = lib().orc_gems_00(self.h, h)
+        buf = (ctypes.c_int * max(1, 2 * n))()
+        lib().orc_bas_00(self.h, h, buf)
+        return [(buf[2 * a], buf[2 * a + 1]) for a in range(n)]
 
     def trip_aab(self, h):
         return lib().orc_trip_aab(self.h, h)

@@ -45,6 +45,8 @@ struct orc_layout {
     int *gab, *gaa;                   /* amo*amo */
     int gems_ab[8], gems_aa[8];
     int *bab[8], *baa[8];             /* pairs (first, second) */
+    int gems_00[8];                   /* singlet geminals (SPIN_ADAPT_Q2) */
+    int *b00[8];                      /* their pairs (i, j), energy order */
     int trip_aab[8], trip_aba[8], trip_aaa[8];
     int *baab[8], *baba[8], *baaa[8]; /* triplets (i, j, k) */
     long off[ORC_OFF_COUNT][8];
@@ -89,6 +91,17 @@ orc_layout *orc_layout_create(const orc_options *opt)
         for (int i = 0; i < opt->amopi[h]; i++) L->sym[amo++] = h;
     }
     if (amo > ORC_MAXAMO) { free(L); return NULL; }
+    /* spin adaptation is refused where the reference refuses it
+     * (v2rdm_solver.cc:538-561: with T1 / T2, and for M_s > 0) and where its
+     * rows would address a block it never lays out (the maximal-spin rows
+     * of CONSTRAIN_SPIN read G2ba, spin.cc:250-268, which SPIN_ADAPT_G2
+     * replaces) */
+    if ((opt->spin_adapt_q2 || opt->spin_adapt_g2) &&
+        (opt->constrain_t1 || opt->constrain_t2 || opt->multiplicity > 1)) {
+        free(L);
+        return NULL;
+    }
+    if (opt->spin_adapt_g2 && opt->constrain_g2 && opt->constrain_spin) { free(L); return NULL; }
     L->amo = amo;
     L->na = opt->nalpha;
     L->nb = opt->nbeta;
@@ -117,6 +130,22 @@ orc_layout *orc_layout_create(const orc_options *opt)
             }
         L->gems_ab[h] = nab;
         L->gems_aa[h] = naa;
+    }
+    /* singlet geminals: energy-ordered pairs (ieo outer, jeo <= ieo inner),
+     * stored (i(ieo), j(jeo)); basis.cc:626-651 */
+    for (int h = 0; h < L->nirrep; h++) {
+        L->b00[h] = (int *)malloc(sizeof(int) * (amo * (amo + 1) + 2));
+        L->gems_00[h] = 0;
+    }
+    for (int ie = 0; ie < amo; ie++) {
+        int i = opt->energy_order ? opt->energy_order[ie] : ie;
+        for (int je = 0; je <= ie; je++) {
+            int j = opt->energy_order ? opt->energy_order[je] : je;
+            int h = L->sym[i] ^ L->sym[j];
+            int n00 = L->gems_00[h]++;
+            L->b00[h][2 * n00] = i;
+            L->b00[h][2 * n00 + 1] = j;
+        }
     }
     /* triplets: basis.cc:655-757 (T1 and T2); aba: all (i,j,k), aab: i < j,
      * aaa: i < j < k, each in (i outer, k inner) order */
@@ -166,15 +195,25 @@ orc_layout *orc_layout_create(const orc_options *opt)
     BLOCKS(ORC_OFF_D1B, n[h])
     BLOCKS(ORC_OFF_Q1A, n[h])
     BLOCKS(ORC_OFF_Q1B, n[h])
-    if (opt->constrain_q2) {
+    if (opt->constrain_q2 && !opt->spin_adapt_q2) {
         BLOCKS(ORC_OFF_Q2AB, g[h])
         BLOCKS(ORC_OFF_Q2AA, ga[h])
         BLOCKS(ORC_OFF_Q2BB, ga[h])
+    } else if (opt->constrain_q2) {      /* v2rdm_solver.cc:597-609 */
+        BLOCKS(ORC_OFF_Q2S, L->gems_00[h])
+        BLOCKS(ORC_OFF_Q2T, ga[h])
+        BLOCKS(ORC_OFF_Q2TP1, ga[h])
+        BLOCKS(ORC_OFF_Q2TM1, ga[h])
     }
-    if (opt->constrain_g2) {
+    if (opt->constrain_g2 && !opt->spin_adapt_g2) {
         BLOCKS(ORC_OFF_G2AB, g[h])
         BLOCKS(ORC_OFF_G2BA, g[h])
         BLOCKS(ORC_OFF_G2AA, 2 * g[h])
+    } else if (opt->constrain_g2) {      /* v2rdm_solver.cc:623-635 */
+        BLOCKS(ORC_OFF_G2S, g[h])
+        BLOCKS(ORC_OFF_G2T, g[h])
+        BLOCKS(ORC_OFF_G2TP1, g[h])
+        BLOCKS(ORC_OFF_G2TM1, g[h])
     }
     if (opt->constrain_t1) {             /* v2rdm_solver.cc:809-825 */
         BLOCKS(ORC_OFF_T1AAA, L->trip_aaa[h])
@@ -220,15 +259,27 @@ orc_layout *orc_layout_create(const orc_options *opt)
         }
         if (opt->constrain_g2) add_family(L, "spin_maxspin", &r, 2L * g[0]);
     }
-    if (opt->constrain_q2) {
+    if (opt->constrain_q2 && !opt->spin_adapt_q2) {
         add_family(L, "q2ab", &r, sab);
         add_family(L, "q2aa", &r, saa);
         add_family(L, "q2bb", &r, saa);
+    } else if (opt->constrain_q2) {      /* v2rdm_solver.cc:976-989 */
+        long s00 = 0;
+        for (int h = 0; h < L->nirrep; h++) s00 += (long)L->gems_00[h] * L->gems_00[h];
+        add_family(L, "q2s", &r, s00);
+        add_family(L, "q2t", &r, saa);
+        add_family(L, "q2t_p1", &r, saa);
+        add_family(L, "q2t_m1", &r, saa);
     }
-    if (opt->constrain_g2) {
+    if (opt->constrain_g2 && !opt->spin_adapt_g2) {
         add_family(L, "g2ab", &r, sab);
         add_family(L, "g2ba", &r, sab);
         add_family(L, "g2aa", &r, 4 * sab);
+    } else if (opt->constrain_g2) {      /* v2rdm_solver.cc:1003-1015 */
+        add_family(L, "g2s", &r, sab);
+        add_family(L, "g2t", &r, sab);
+        add_family(L, "g2t_p1", &r, sab);
+        add_family(L, "g2t_m1", &r, sab);
     }
     if (opt->constrain_t1) {             /* t1.cc:433-809 row order: aab, bba, aaa, bbb */
         long s1 = 0, s3 = 0;
@@ -262,6 +313,7 @@ void orc_layout_destroy(orc_layout *L)
     for (int h = 0; h < L->nirrep; h++) {
         free(L->bab[h]); free(L->baa[h]);
         free(L->baab[h]); free(L->baba[h]); free(L->baaa[h]);
+        free(L->b00[h]);
     }
     free(L->gab); free(L->gaa); free(L->bdim); free(L->boff);
     free(L);
@@ -275,6 +327,11 @@ long orc_offset(const orc_layout *L, int which, int h) { return L->off[which][h]
 int orc_gems_ab(const orc_layout *L, int h) { return L->gems_ab[h]; }
 int orc_gems_aa(const orc_layout *L, int h) { return L->gems_aa[h]; }
 int orc_trip_aab(const orc_layout *L, int h) { return L->trip_aab[h]; }
+int orc_gems_00(const orc_layout *L, int h) { return L->gems_00[h]; }
+void orc_bas_00(const orc_layout *L, int h, int *pairs)
+{
+    memcpy(pairs, L->b00[h], sizeof(int) * 2 * L->gems_00[h]);
+}
 int orc_trip_aba(const orc_layout *L, int h) { return L->trip_aba[h]; }
 int orc_trip_aaa(const orc_layout *L, int h) { return L->trip_aaa[h]; }
 int orc_ibas_ab(const orc_layout *L, int p, int q) { return L->gab[p * L->amo + q]; }
@@ -613,7 +670,9 @@ void orc_build_b(const orc_layout *L, double *b)
         }
         if (L->o.constrain_g2) off += 2L * L->gems_ab[0];
     }
-    if (L->o.constrain_q2) {
+    /* the spin-adapted Q2 rows carry no constant: b = 0 there
+     * (v2rdm_solver.cc:2709-2745) */
+    if (L->o.constrain_q2 && !L->o.spin_adapt_q2) {
         for (int h = 0; h < L->nirrep; h++) {
             int g = L->gems_ab[h];
             for (int ij = 0; ij < g; ij++) {

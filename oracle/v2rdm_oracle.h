@@ -14,6 +14,8 @@
  *   orc_Au / orc_ATu    v2rdm_solver.cc:3020-3065 / 3114-3158 dispatch over
  *                       d2.cc:296-547 / 58-293, spin.cc:274-493 / 58-271,
  *                       q2.cc:801-950 / 953-1074, g2.cc:1061-1260 / 1263-1457,
+ *                       spin-adapted q2.cc:260-467 / 470-670,
+ *                       g2.cc:235-546 / 549-856,
  *                       t1.cc:433-809 / 812-1173,
  *                       t2.cc:1251-1861 / 3399-3993 (the *_slow maps)
  *   orc_cg_Ax           v2rdm_solver.cc:3207-3213
@@ -41,6 +43,14 @@ typedef struct orc_options {
     int constrain_q2, constrain_g2, constrain_t2;
     int constrain_spin, constrain_sz;
     int constrain_t1;    /* POSITIVITY DQGT1 / DQGT1T2                        */
+    int spin_adapt_q2;   /* SPIN_ADAPT_Q2 (plugin.cc:110): Q2s, Q2t, Q2t(+1),  */
+                         /* Q2t(-1) blocks instead of Q2ab, Q2aa, Q2bb       */
+    int spin_adapt_g2;   /* SPIN_ADAPT_G2 (plugin.cc:108): G2s, G2t, G2t(+1),  */
+                         /* G2t(-1) blocks instead of G2ab, G2ba, G2aa       */
+    /* active orbitals (Pitzer index) from lowest to highest orbital energy:
+     * the order of the singlet geminals of SPIN_ADAPT_Q2 (basis.cc:626-651);
+     * NULL = Pitzer order */
+    const int *energy_order;
 } orc_options;
 
 typedef struct orc_layout orc_layout;
@@ -60,6 +70,10 @@ int  orc_gems_aa(const orc_layout *L, int h);
 int  orc_trip_aab(const orc_layout *L, int h);
 int  orc_trip_aba(const orc_layout *L, int h);
 int  orc_trip_aaa(const orc_layout *L, int h);
+/* singlet geminals of SPIN_ADAPT_Q2: count, and the pairs (i, j) of irrep h
+ * in their order (pairs has 2 * orc_gems_00(L, h) entries) */
+int  orc_gems_00(const orc_layout *L, int h);
+void orc_bas_00(const orc_layout *L, int h, int *pairs);
 /* geminal tables: ab index of (p,q); aa index of (p,q) (-1 if p==q) */
 int  orc_ibas_ab(const orc_layout *L, int p, int q);
 int  orc_ibas_aa(const orc_layout *L, int p, int q);
@@ -76,6 +90,8 @@ enum {
     ORC_OFF_G2AB, ORC_OFF_G2BA, ORC_OFF_G2AA,
     ORC_OFF_T2AAA, ORC_OFF_T2BBB, ORC_OFF_T2AAB, ORC_OFF_T2BBA,
     ORC_OFF_T1AAA, ORC_OFF_T1BBB, ORC_OFF_T1AAB, ORC_OFF_T1BBA,
+    ORC_OFF_Q2S, ORC_OFF_Q2T, ORC_OFF_Q2TP1, ORC_OFF_Q2TM1,
+    ORC_OFF_G2S, ORC_OFF_G2T, ORC_OFF_G2TP1, ORC_OFF_G2TM1,
     ORC_OFF_COUNT
 };
 

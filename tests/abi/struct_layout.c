@@ -17,6 +17,8 @@ int main(void)
     F(v2g_layout_desc, constrain_q2); F(v2g_layout_desc, constrain_g2);
     F(v2g_layout_desc, constrain_t2); F(v2g_layout_desc, constrain_spin);
     F(v2g_layout_desc, constrain_sz); F(v2g_layout_desc, constrain_t1);
+    F(v2g_layout_desc, spin_adapt_q2); F(v2g_layout_desc, spin_adapt_g2);
+    F(v2g_layout_desc, energy_order);
     S(v2g_params);
     F(v2g_params, tau); F(v2g_params, r_convergence); F(v2g_params, e_convergence);
     F(v2g_params, cg_convergence); F(v2g_params, maxiter); F(v2g_params, cg_maxiter);

@@ -24,41 +24,9 @@ struct Emu {
     long long violations = 0;
     explicit Emu(const Options &o) : L(o)
     {
-        D.nirrep = L.nirrep;
-        D.amo = L.amo;
-        D.singlet = L.singlet;
-        D.sz = o.sz;
-        D.spin = o.spin;
-        D.q2 = o.q2;
-        D.g2 = o.g2;
-        D.t2 = o.t2;
-        D.t1 = o.t1;
-        D.na = L.na;
-        D.nb = L.nb;
-        for (int h = 0; h < 8; h++) {
-            D.amopi[h] = o.amopi[h];
-            D.poff[h] = L.poff[h];
-            D.gems_ab[h] = L.gems_ab[h];
-            D.gems_aa[h] = L.gems_aa[h];
-            D.trip_aab[h] = L.trip_aab[h];
-            D.trip_aba[h] = L.trip_aba[h];
-            D.trip_aaa[h] = L.trip_aaa[h];
-            D.baaa_off[h] = L.baaa_off[h];
-            D.bab_off[h] = L.bab_off[h];
-            D.baa_off[h] = L.baa_off[h];
-            D.baab_off[h] = L.baab_off[h];
-            D.baba_off[h] = L.baba_off[h];
-        }
-        std::memcpy(D.xoff, L.xoff, sizeof(D.xoff));
-        std::memcpy(D.rowoff, L.rowoff, sizeof(D.rowoff));
-        D.sym = L.sym.data();
-        D.gab = L.gab.data();
-        D.gaa = L.gaa.data();
-        D.bab = L.bab.data();
-        D.baa = L.baa.data();
-        D.baab = L.baab.data();
-        D.baba = L.baba.data();
-        D.baaa = L.baaa.data();
+        D = make_dev_layout(L, {L.sym.data(), L.gab.data(), L.gaa.data(), L.bab.data(),
+                                L.baa.data(), L.baab.data(), L.baba.data(), L.baaa.data(),
+                                L.b00.data()});
     }
 };
 
@@ -89,7 +57,7 @@ struct CheckedScatter {
 extern "C" {
 
 void *emu_create(int nirrep, const int *amopi, int na, int nb, int mult, int q2, int g2, int t2,
-                 int spin, int sz, int t1)
+                 int spin, int sz, int t1, int sa_q2, int sa_g2, const int *eorder)
 {
     Options o;
     o.nirrep = nirrep;
@@ -103,6 +71,13 @@ void *emu_create(int nirrep, const int *amopi, int na, int nb, int mult, int q2,
     o.t1 = t1;
     o.spin = spin;
     o.sz = sz;
+    o.sa_q2 = sa_q2;
+    o.sa_g2 = sa_g2;
+    if (eorder) {
+        int amo = 0;
+        for (int h = 0; h < nirrep; h++) amo += o.amopi[h];
+        o.eorder.assign(eorder, eorder + amo);
+    }
     try {
         return new Emu(o);
     } catch (...) {
@@ -188,7 +163,7 @@ void emu_ATu(void *h, const double *y, double *ATy)
             const double yv = y[sg.aux + e];
             ATy[i] = -yv;
             CheckedScatter f{ATy, dimx, &E->violations, yv};
-            rows::triplet_terms(E->D, sg.kind, sg.sub, sg.h, e, f, false);
+            rows::scatter_terms(E->D, sg.kind, sg.sub, sg.h, e, f, false);
         }
     }
     for (int64_t i = 0; i < dimx; i++)

@@ -111,6 +111,12 @@ def exact_x(F, psi, L):
       Q2ab(ij,kl)= <a_ia a_jb a+_lb a+_ka>    : vectors a+_jb a+_ia psi
       G2ab(ij,kl)= <a+_ia a_jb a+_lb a_ka>    : vectors a+_jb a_ia psi
       G2 super block: vectors [a+_ja a_ia ; a+_jb a_ib] psi
+      spin-adapted Q2: Q2s over singlet geminals (i, j) (i = j included):
+                   (a+_jb a+_ia + a+_ib a+_ja) psi / sqrt2; Q2t over aa
+                   geminals: (a+_jb a+_ia - a+_ib a+_ja) psi / sqrt2;
+                   Q2t(+1) / Q2t(-1) = Q2aa / Q2bb
+      spin-adapted G2: G2s / G2t: (a+_ja a_ia +- a+_jb a_ib) psi / sqrt2;
+                   G2t(+1) / G2t(-1) = G2ab / G2ba
       T1 blocks  = <B+ B> + <B B+>, B_ijk = a_k a_j a_i (aab: i<j alpha, k beta;
                    aaa: i<j<k), i.e. the D3 + Q3 matrix
       T2 blocks  = <B+ B> + <B B+> for B = the triplet operators of each block
@@ -171,10 +177,20 @@ def exact_x(F, psi, L):
                 vt = [(a(q, 1) @ (a(p, 0) @ psi) - a(p, 1) @ (a(q, 0) @ psi)) / np.sqrt(2.0)
                       for p, q in P]
                 put("D200", h, gram(vs + vt))
-        if L.opts.constrain_q2:
+        sa_q2 = getattr(L.opts, "spin_adapt_q2", 0)
+        sa_g2 = getattr(L.opts, "spin_adapt_g2", 0)
+        if L.opts.constrain_q2 and not sa_q2:
             vq = [c(q, 1) @ (c(p, 0) @ psi) for p, q in P]
             put("Q2AB", h, gram(vq))
-        if L.opts.constrain_g2:
+        if L.opts.constrain_q2 and sa_q2:
+            # singlet hole pairs (a+_jb a+_ia + a+_ib a+_ja) / sqrt2 over the
+            # singlet geminals (i, j), i = j included
+            P00 = L.bas_00(h)
+            if P00:
+                vs = [(c(j, 1) @ (c(i, 0) @ psi) + c(i, 1) @ (c(j, 0) @ psi)) / np.sqrt(2.0)
+                      for i, j in P00]
+                put("Q2S", h, gram(vs))
+        if L.opts.constrain_g2 and not sa_g2:
             vg = [c(q, 1) @ (a(p, 0) @ psi) for p, q in P]      # a+_jb a_ia psi
             put("G2AB", h, gram(vg))
             vg2 = [c(q, 0) @ (a(p, 1) @ psi) for p, q in P]     # a+_ja a_ib psi
@@ -182,16 +198,32 @@ def exact_x(F, psi, L):
             vaa = [c(q, 0) @ (a(p, 0) @ psi) for p, q in P]
             vbb = [c(q, 1) @ (a(p, 1) @ psi) for p, q in P]
             put("G2AA", h, gram(vaa + vbb))
+        if L.opts.constrain_g2 and sa_g2:
+            # M_s = 0 singlet / triplet particle-hole pairs
+            # (a+_ja a_ia +- a+_jb a_ib) / sqrt2, and the spin-flip pairs
+            vaa = [c(q, 0) @ (a(p, 0) @ psi) for p, q in P]
+            vbb = [c(q, 1) @ (a(p, 1) @ psi) for p, q in P]
+            put("G2S", h, gram([(u + v) / np.sqrt(2.0) for u, v in zip(vaa, vbb)]))
+            put("G2T", h, gram([(u - v) / np.sqrt(2.0) for u, v in zip(vaa, vbb)]))
+            put("G2TP1", h, gram([c(q, 1) @ (a(p, 0) @ psi) for p, q in P]))
+            put("G2TM1", h, gram([c(q, 0) @ (a(p, 1) @ psi) for p, q in P]))
     for h in range(L.nirrep):
         P = aa_pairs(h)
         if not P:
             continue
+        sa_q2 = getattr(L.opts, "spin_adapt_q2", 0)
         for s, (dn, qn) in enumerate((("D2AA", "Q2AA"), ("D2BB", "Q2BB"))):
             vd = [a(q, s) @ (a(p, s) @ psi) for p, q in P]
             put(dn, h, gram(vd))
             if L.opts.constrain_q2:
+                # like-spin hole pairs: Q2aa / Q2bb, or Q2t(+1) / Q2t(-1)
                 vq = [c(q, s) @ (c(p, s) @ psi) for p, q in P]
-                put(qn, h, gram(vq))
+                put(("Q2TM1" if s else "Q2TP1") if sa_q2 else qn, h, gram(vq))
+        if L.opts.constrain_q2 and sa_q2:
+            # M_s = 0 triplet hole pairs (a+_jb a+_ia - a+_ib a+_ja) / sqrt2
+            vt = [(c(q, 1) @ (c(p, 0) @ psi) - c(p, 1) @ (c(q, 0) @ psi)) / np.sqrt(2.0)
+                  for p, q in P]
+            put("Q2T", h, gram(vt))
     def anti(ops):
         """<B+ B> + <B B+> as Gram of B psi and B+ psi."""
         v1 = [op @ psi for op in ops]

@@ -6,7 +6,9 @@ The batched GPU solver is checked point by point against them.
 
 Run from the repository root (points given on the command line are
 recomputed and merged):  python tests/golden/make_scan_energies.py [index ...]
-The H16 point takes the longest (tens of minutes on 8 cores)."""
+Stretched chains converge slowly: points 2, 3, 5 and 6 took 52 000 - 134 000
+outer iterations and 10 - 30 minutes each on 8 cores; points 0 and 1 (H16 at
+2.57 A, H14 at 2.80 A) need hours each and are not in the file."""
 import json
 import os
 import sys

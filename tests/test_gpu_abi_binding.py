@@ -10,7 +10,6 @@ import subprocess
 import numpy as np
 import pytest
 
-import oracle_py as orc
 
 pytestmark = pytest.mark.gpu
 
@@ -18,13 +17,12 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 EXE = os.path.join(HERE, "abi", "build", "plugin_binding")
 
 
-@pytest.mark.parametrize("amopi,na,nb", [([2, 1, 1, 0], 2, 2), ([4], 2, 2)])
-def test_cpp_binding_matches_oracle(tmp_path, amopi, na, nb):
-    import fci_tools as ft
+@pytest.mark.parametrize("key", ["d2h_2110_s0", "c1_4_s0"])
+def test_cpp_binding_matches_oracle(tmp_path, key):
+    import oracle_loops as ol
     assert os.path.exists(EXE), "build it with __graft_entry__.build() (make -C tests/abi)"
-    L = orc.Layout(amopi, na, nb, None, "DQG")
-    irreps = [L.irrep(p) for p in range(L.amo)]
-    h1, eri = ft.random_hamiltonian(L.amo, 0, irreps if len(amopi) > 1 else None)
+    amopi, na, nb, _, _, _ = ol.CASES[key]
+    L, h1, eri = ol.inputs(key)
     ints = tmp_path / "ints.bin"
     out = tmp_path / "out.bin"
     np.concatenate([h1.ravel(), eri.ravel()]).astype(np.float64).tofile(ints)
@@ -34,9 +32,8 @@ def test_cpp_binding_matches_oracle(tmp_path, amopi, na, nb):
     assert res.returncode == 0, res.stderr
     rec = json.loads(res.stdout.strip().splitlines()[-1])
     assert (rec["dimx"], rec["ncon"]) == (L.dimx, L.ncon)
-    c, b = L.build_c(h1, eri), L.build_b()
-    x0, z0, y0 = L.guess_random()
-    ref = L.bpsdp(c, b, x0, z0, y0, r_conv=1e-6, e_conv=1e-6)
+    c = L.build_c(h1, eri)
+    ref = ol.converged(key)     # oracle solve at R/E_CONVERGENCE 1e-6
     assert rec["converged"] == 1 and ref["converged"]
     assert abs(rec["energy"] - ref["energy_primal"]) < 1e-6
     data = np.fromfile(out, dtype=np.float64)

@@ -176,8 +176,8 @@ def test_random_guess_is_glibc_rand_sequence():
     assert x.min() >= -2.0 and x.max() <= 0.0
 
 
-def _loop_energy(amopi, na, nb, pos, seed=0, conv=1e-5):
-    L = orc.Layout(amopi, na, nb, None, pos)
+def _loop_energy(amopi, na, nb, pos, seed=0, conv=1e-5, **kw):
+    L = orc.Layout(amopi, na, nb, None, pos, **kw)
     h, eri = _ham(L, seed)
     F = ft.Fock(L.amo)
     E, _ = F.ground_state(h, eri, na, nb, s_target=(na - nb) / 2)
@@ -209,8 +209,9 @@ def test_two_holes_dqg_is_exact():
 def test_resumed_loop_is_the_same_iteration():
     """A solve split into two calls (the second resuming mu, the iteration
     count and the errors of the first) walks the same trajectory as one call:
-    same outer iteration count (to one iteration: the OpenMP reductions do
-    not fix their summation order) and energy."""
+    same outer iteration count (to a few iterations in 7 000: the OpenMP
+    reductions do not fix their summation order, so two runs drift apart in
+    the last bits) and energy."""
     L = orc.Layout([4], 2, 2, None, "DQG")
     h, eri = _ham(L, 0)
     c, b = L.build_c(h, eri), L.build_b()
@@ -220,7 +221,7 @@ def test_resumed_loop_is_the_same_iteration():
     assert first["oiter"] == 150 and not first["converged"]
     second = L.bpsdp(c, b, x, z, y, maxiter=20000, resume=first)
     assert one["converged"] and second["converged"]
-    assert abs(second["oiter"] - one["oiter"]) <= 2
+    assert abs(second["oiter"] - one["oiter"]) <= max(2, one["oiter"] // 500)
     assert abs(second["energy_primal"] - one["energy_primal"]) < 1e-8
 
 
@@ -252,3 +253,138 @@ def test_mlp_gradient_matches_finite_differences():
             flat[k] = old
             fd = (lp - lm) / (2 * eps)
             assert abs(fd - g[pi][k]) < 2e-3 * (1 + abs(fd)), (pi, k)
+
+
+# ---------------------------------------------------------------------------
+# spin-adapted Q2 / G2 (SPIN_ADAPT_Q2 / _G2, q2.cc:260-670, g2.cc:235-856)
+# ---------------------------------------------------------------------------
+
+SA_LAYOUTS = [
+    # amopi, nalpha, nbeta, positivity, constrain_spin, sa_q2, sa_g2, energy order
+    ([4], 2, 2, "DQG", True, True, False, None),
+    ([4], 2, 2, "DQG", True, True, False, [2, 0, 3, 1]),
+    ([4], 2, 2, "DQG", False, True, True, [1, 3, 0, 2]),
+    ([2, 1, 1, 0], 2, 2, "DQG", False, True, True, [0, 2, 1, 3]),
+    ([1, 1, 1, 1, 0, 1, 0, 0], 2, 2, "DQ", True, True, False, [4, 0, 1, 2, 3]),
+    ([5], 3, 3, "DG", False, False, True, None),
+]
+
+
+def _sa_layout(amopi, na, nb, pos, spin, saq, sag, eo):
+    return orc.Layout(amopi, na, nb, 1, pos, constrain_spin=spin, spin_adapt_q2=saq,
+                      spin_adapt_g2=sag, energy_order=eo)
+
+
+@pytest.mark.parametrize("amopi,na,nb,pos,spin,saq,sag,eo", SA_LAYOUTS)
+def test_exact_rdms_are_feasible_spin_adapted(amopi, na, nb, pos, spin, saq, sag, eo):
+    """The spin-adapted blocks of an FCI singlet, each built as the Gram
+    matrix of its singlet / triplet pair operators, satisfy every row."""
+    L = _sa_layout(amopi, na, nb, pos, spin, saq, sag, eo)
+    h, eri = _ham(L)
+    F = ft.Fock(L.amo)
+    E, psi = F.ground_state(h, eri, na, nb, s_target=0)
+    x = ft.exact_x(F, psi, L)
+    r = L.Au(x) - L.build_b()
+    for name, st, cnt in L.row_families():
+        assert np.abs(r[st:st + cnt]).max() < 1e-11, name
+    assert abs(L.build_c(h, eri) @ x - E) < 1e-10
+
+
+def test_spin_adapted_feasibility_detects_planted_errors():
+    """Each planted mistake breaks its family's rows: the triplet sign of Q2t,
+    singlet and triplet particle-hole blocks swapped, and the weight of the
+    i = j singlet geminals (the pair operator of i = j is sqrt2 a+_ib a+_ia)."""
+    L = _sa_layout([4], 2, 2, "DQG", False, True, True, [1, 3, 0, 2])
+    h, eri = _ham(L)
+    F = ft.Fock(L.amo)
+    E, psi = F.ground_state(h, eri, 2, 2, s_target=0)
+    fam = {name: (st, cnt) for name, st, cnt in L.row_families()}
+
+    def resid(x, rows):
+        st, cnt = fam[rows]
+        return np.abs((L.Au(x) - L.build_b())[st:st + cnt]).max()
+
+    x0 = ft.exact_x(F, psi, L)
+    assert resid(x0, "q2t") < 1e-11 and resid(x0, "g2s") < 1e-11 and resid(x0, "q2s") < 1e-11
+    x = x0.copy()
+    o, g = L.offset("Q2T", 0), L.gems_aa(0)
+    x[o:o + g * g] *= -1.0
+    assert resid(x, "q2t") > 1e-3
+    x = x0.copy()
+    g = L.gems_ab(0)
+    os_, ot = L.offset("G2S", 0), L.offset("G2T", 0)
+    x[os_:os_ + g * g], x[ot:ot + g * g] = x0[ot:ot + g * g], x0[os_:os_ + g * g]
+    assert resid(x, "g2s") > 1e-3
+    x = x0.copy()
+    g0 = L.gems_00(0)
+    o = L.offset("Q2S", 0)
+    diag = [a for a, (i, j) in enumerate(L.bas_00(0)) if i == j]
+    for a in diag:
+        for b_ in range(g0):
+            x[o + a * g0 + b_] /= np.sqrt(2.0)
+            x[o + b_ * g0 + a] /= np.sqrt(2.0)
+    assert resid(x, "q2s") > 1e-3
+
+
+def test_singlet_geminals_follow_the_energy_order():
+    """Convention (basis.cc:626-651, any order is mathematically valid): the
+    singlet geminals run over orbital-energy ranks, higher rank outer, pair
+    (orbital of the higher rank, orbital of the lower); None = Pitzer order."""
+    eo = [2, 0, 3, 1]
+    L = _sa_layout([2, 2], 2, 2, "DQG", True, True, False, eo)
+    sym = [L.irrep(p) for p in range(4)]
+    for h in range(2):
+        want = [(eo[a], eo[b]) for a in range(4) for b in range(a + 1) if sym[eo[a]] ^ sym[eo[b]] == h]
+        assert L.bas_00(h) == want
+    P = _sa_layout([3], 2, 2, "DQG", True, True, False, None)
+    assert P.bas_00(0) == [(a, b) for a in range(3) for b in range(a + 1)]
+    assert P.gems_00(0) == 6
+
+
+@pytest.mark.parametrize("amopi,na,nb,pos,spin,saq,sag,eo", SA_LAYOUTS)
+def test_adjointness_spin_adapted(amopi, na, nb, pos, spin, saq, sag, eo):
+    L = _sa_layout(amopi, na, nb, pos, spin, saq, sag, eo)
+    rng = np.random.default_rng(2)
+    x = rng.normal(size=L.dimx)
+    y = rng.normal(size=L.ncon)
+    lhs = L.Au(x) @ y
+    rhs = x @ L.ATu(y)
+    assert abs(lhs - rhs) < 1e-12 * (abs(lhs) + np.linalg.norm(x) * np.linalg.norm(y))
+
+
+def test_spin_adapted_sizes():
+    """dimx / ncon of the spin-adapted layouts from their block counts
+    (v2rdm_solver.cc:597-635, 976-1015): Q2 trades gab^2 + 2 gaa^2 for
+    g00^2 + 3 gaa^2, G2 trades 6 gab^2 for 4 gab^2."""
+    P = orc.Layout([3, 1, 2, 0], 3, 3, 1, "DQG", constrain_spin=False)
+    Q = orc.Layout([3, 1, 2, 0], 3, 3, 1, "DQG", constrain_spin=False, spin_adapt_q2=True,
+                   spin_adapt_g2=True)
+    dq = sum(Q.gems_00(h) ** 2 + Q.gems_aa(h) ** 2 - Q.gems_ab(h) ** 2 for h in range(4))
+    dg = sum(-2 * Q.gems_ab(h) ** 2 for h in range(4))
+    assert Q.dimx - P.dimx == dq + dg and Q.ncon - P.ncon == dq + dg
+    assert sum(Q.gems_00(h) for h in range(4)) == 6 * 7 // 2
+
+
+def test_two_holes_dqg_is_exact_with_spin_adapted_q2():
+    E, st = _loop_energy([4], 3, 3, "DQG", spin_adapt_q2=True, energy_order=[3, 1, 0, 2])
+    assert st["converged"]
+    assert abs(st["energy_primal"] - E) < 1e-4
+
+
+def test_spin_adapted_energies_bracket():
+    """Spin-adapted Q2 is the plain Q2 condition in a rotated basis (same
+    energy); spin-adapted G2 keeps only the singlet and triplet projections of
+    the like-spin particle-hole block (a relaxation: energy not above the
+    plain one, both above the D-only bound and below FCI)."""
+    kw = dict(seed=3, conv=1e-5)
+    E, plain = _loop_energy([4], 2, 2, "DQG", **kw)
+    _, saq = _loop_energy([4], 2, 2, "DQG", spin_adapt_q2=True, **kw)
+    assert plain["converged"] and saq["converged"]
+    assert abs(saq["energy_primal"] - plain["energy_primal"]) < 2e-5
+    _, nospin = _loop_energy([4], 2, 2, "DQG", constrain_spin=False, **kw)
+    _, sag = _loop_energy([4], 2, 2, "DQG", constrain_spin=False, spin_adapt_g2=True, **kw)
+    _, donly = _loop_energy([4], 2, 2, "D", constrain_spin=False, **kw)
+    assert sag["converged"] and nospin["converged"] and donly["converged"]
+    assert sag["energy_primal"] <= nospin["energy_primal"] + 2e-5
+    assert donly["energy_primal"] <= sag["energy_primal"] + 2e-5
+    assert sag["energy_primal"] <= E + 2e-5

@@ -17,7 +17,8 @@ class LayoutDesc(ctypes.Structure):
                 ("multiplicity", ctypes.c_int), ("constrain_q2", ctypes.c_int),
                 ("constrain_g2", ctypes.c_int), ("constrain_t2", ctypes.c_int),
                 ("constrain_spin", ctypes.c_int), ("constrain_sz", ctypes.c_int),
-                ("constrain_t1", ctypes.c_int)]
+                ("constrain_t1", ctypes.c_int), ("spin_adapt_q2", ctypes.c_int),
+                ("spin_adapt_g2", ctypes.c_int), ("energy_order", ctypes.POINTER(ctypes.c_int))]
 
 
 class Params(ctypes.Structure):

@@ -14,11 +14,13 @@ import torch
 
 from . import _lib
 from ._lib import check
-from .solver import POSITIVITY, _ptr
+from .solver import _ptr, fill_desc
 
 
 class BatchSolver:
-    """problems: list of dict(amopi, nalpha, nbeta[, multiplicity][, positivity])."""
+    """problems: list of dict(amopi, nalpha, nbeta[, multiplicity][, positivity]
+    [, constrain_spin][, constrain_sz][, spin_adapt_q2][, spin_adapt_g2]
+    [, energy_order]), the keywords of V2RDMSolver."""
 
     def __init__(self, problems, device="cuda:0", options=None):
         self.lib = _lib.load()
@@ -27,16 +29,13 @@ class BatchSolver:
         if n == 0:
             raise ValueError("empty batch")
         descs = (_lib.LayoutDesc * n)()
+        self._eorders = []
         for d, pb in zip(descs, problems):
-            amopi = list(pb["amopi"])
-            pos = pb.get("positivity", "DQG").upper()
-            d.nirrep = len(amopi)
-            for h in range(8):
-                d.amopi[h] = amopi[h] if h < len(amopi) else 0
-            d.nalpha, d.nbeta = int(pb["nalpha"]), int(pb["nbeta"])
-            d.multiplicity = int(pb.get("multiplicity") or pb["nalpha"] - pb["nbeta"] + 1)
-            d.constrain_q2, d.constrain_g2, d.constrain_t1, d.constrain_t2 = POSITIVITY[pos]
-            d.constrain_spin, d.constrain_sz = 1, 1
+            self._eorders.append(fill_desc(
+                d, pb["amopi"], pb["nalpha"], pb["nbeta"], pb.get("multiplicity"),
+                pb.get("positivity", "DQG"), pb.get("constrain_spin", True),
+                pb.get("constrain_sz", True), pb.get("spin_adapt_q2", False),
+                pb.get("spin_adapt_g2", False), pb.get("energy_order")))
         self.problems = problems
         self.nprob = n
         with torch.cuda.device(self.device):

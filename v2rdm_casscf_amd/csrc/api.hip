@@ -48,7 +48,7 @@ struct v2g_solver {
     std::unique_ptr<v2g::Layout> L;
     hipStream_t own = nullptr, st = nullptr;
     // layout on device
-    DevBuf<int> d_sym, d_gab, d_gaa, d_bab, d_baa, d_baab, d_baba, d_baaa;
+    DevBuf<int> d_sym, d_gab, d_gaa, d_bab, d_baa, d_baab, d_baba, d_baaa, d_b00;
     DevBuf<v2g::DevLayout> d_lay;
     DevBuf<v2g::Segment> d_rowsegs, d_xsegs, d_t2segs;
     DevBuf<int64_t> d_rowtiles, d_xtiles, d_t2tiles;
@@ -173,19 +173,7 @@ struct v2g_solver {
 
 v2g_solver::v2g_solver(const v2g_layout_desc &d)
 {
-    v2g::Options o;
-    o.nirrep = d.nirrep;
-    for (int h = 0; h < 8; h++) o.amopi[h] = h < d.nirrep ? d.amopi[h] : 0;
-    o.nalpha = d.nalpha;
-    o.nbeta = d.nbeta;
-    o.multiplicity = d.multiplicity;
-    o.q2 = d.constrain_q2;
-    o.g2 = d.constrain_g2;
-    o.t2 = d.constrain_t2;
-    o.t1 = d.constrain_t1;
-    o.spin = d.constrain_spin;
-    o.sz = d.constrain_sz;
-    L.reset(new v2g::Layout(o));
+    L.reset(new v2g::Layout(v2g::options_from_desc(d)));
     if (const char *e = std::getenv("V2G_CG_GRAPHS")) use_graphs = std::atoi(e) != 0;
     HIPCHK(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
     st = own;
@@ -198,43 +186,9 @@ v2g_solver::v2g_solver(const v2g_layout_desc &d)
     d_baab.upload(L->baab);
     d_baba.upload(L->baba);
     d_baaa.upload(L->baaa);
-    v2g::DevLayout dl{};
-    dl.nirrep = L->nirrep;
-    dl.amo = L->amo;
-    dl.singlet = L->singlet;
-    dl.sz = o.sz;
-    dl.spin = o.spin;
-    dl.q2 = o.q2;
-    dl.g2 = o.g2;
-    dl.t2 = o.t2;
-    dl.t1 = o.t1;
-    dl.na = L->na;
-    dl.nb = L->nb;
-    dl.ms = (o.multiplicity - 1.0) / 2.0;
-    for (int h = 0; h < 8; h++) {
-        dl.amopi[h] = o.amopi[h];
-        dl.poff[h] = L->poff[h];
-        dl.gems_ab[h] = L->gems_ab[h];
-        dl.gems_aa[h] = L->gems_aa[h];
-        dl.trip_aab[h] = L->trip_aab[h];
-        dl.trip_aba[h] = L->trip_aba[h];
-        dl.trip_aaa[h] = L->trip_aaa[h];
-        dl.baaa_off[h] = L->baaa_off[h];
-        dl.bab_off[h] = L->bab_off[h];
-        dl.baa_off[h] = L->baa_off[h];
-        dl.baab_off[h] = L->baab_off[h];
-        dl.baba_off[h] = L->baba_off[h];
-    }
-    std::memcpy(dl.xoff, L->xoff, sizeof(dl.xoff));
-    std::memcpy(dl.rowoff, L->rowoff, sizeof(dl.rowoff));
-    dl.sym = d_sym.p;
-    dl.gab = d_gab.p;
-    dl.gaa = d_gaa.p;
-    dl.bab = d_bab.p;
-    dl.baa = d_baa.p;
-    dl.baab = d_baab.p;
-    dl.baba = d_baba.p;
-    dl.baaa = d_baaa.p;
+    d_b00.upload(L->b00);
+    const v2g::DevLayout dl = v2g::make_dev_layout(
+        *L, {d_sym.p, d_gab.p, d_gaa.p, d_bab.p, d_baa.p, d_baab.p, d_baba.p, d_baaa.p, d_b00.p});
     d_lay.upload(std::vector<v2g::DevLayout>{dl});
 
     auto tiles = [](const std::vector<v2g::Segment> &segs, std::vector<int64_t> &ts) {
@@ -274,7 +228,7 @@ v2g_solver::v2g_solver(const v2g_layout_desc &d)
     // enters, and every column must fit 30 bits, for the packed G2 table
     int64_t d1lo = INT64_MAX, d1hi = -1;
     for (int hh = 0; hh < L->nirrep; hh++) {
-        const int64_t nn = (int64_t)o.amopi[hh] * o.amopi[hh];
+        const int64_t nn = (int64_t)L->opt.amopi[hh] * L->opt.amopi[hh];
         if (!nn) continue;
         d1lo = std::min(d1lo, L->xoff[v2g::XB_D1A][hh]);
         d1hi = std::max(d1hi, L->xoff[v2g::XB_D1B][hh] + nn);
@@ -1178,7 +1132,7 @@ int v2g_build_b(v2g_handle h, double *b)
         const double ms = (L.opt.multiplicity - 1.0) / 2.0;
         b[L.rowoff[v2g::RF_S2][0][0]] = 0.5 * (na + nb) + ms * ms - ms * (ms + 1.0);
     }
-    if (L.opt.q2) {
+    if (L.q2_plain()) {  // spin-adapted Q2 rows have b = 0 (v2rdm_solver.cc:2709-2745)
         for (int hh = 0; hh < L.nirrep; hh++) {
             const int64_t g = L.gems_ab[hh], o = L.rowoff[v2g::RF_Q2AB][0][hh];
             for (int64_t ij = 0; ij < g; ij++) b[o + ij * g + ij] = -1.0;

@@ -377,70 +377,18 @@ v2g_batch::v2g_batch(const v2g_layout_desc *descs, int n) : np(n)
     std::vector<v2g::Segment> xsegs, t2segs;
     int64_t g2rows = 0;
     for (int q = 0; q < n; q++) {
-        const v2g_layout_desc &d = descs[q];
-        if (d.nirrep != 1 && d.nirrep != 2 && d.nirrep != 4 && d.nirrep != 8)
-            throw std::invalid_argument("nirrep must be 1, 2, 4 or 8");
-        v2g::Options o;
-        o.nirrep = d.nirrep;
-        for (int h = 0; h < 8; h++) o.amopi[h] = h < d.nirrep ? d.amopi[h] : 0;
-        o.nalpha = d.nalpha;
-        o.nbeta = d.nbeta;
-        o.multiplicity = d.multiplicity;
-        o.q2 = d.constrain_q2;
-        o.g2 = d.constrain_g2;
-        o.t2 = d.constrain_t2;
-        o.t1 = d.constrain_t1;
-        o.spin = d.constrain_spin;
-        o.sz = d.constrain_sz;
-        L.emplace_back(new v2g::Layout(o));
+        L.emplace_back(new v2g::Layout(v2g::options_from_desc(descs[q])));
         const v2g::Layout &Lq = *L.back();
         auto up = [&](const std::vector<int> &v) {
             tables.emplace_back(new DevBuf<int>());
             tables.back()->upload(v.empty() ? std::vector<int>{0} : v);
-            return tables.back()->p;
+            return (const int *)tables.back()->p;
         };
-        v2g::DevLayout dl{};
-        dl.nirrep = Lq.nirrep;
-        dl.amo = Lq.amo;
-        dl.singlet = Lq.singlet;
-        dl.sz = o.sz;
-        dl.spin = o.spin;
-        dl.q2 = o.q2;
-        dl.g2 = o.g2;
-        dl.t2 = o.t2;
-        dl.t1 = o.t1;
-        dl.na = Lq.na;
-        dl.nb = Lq.nb;
-        dl.ms = (o.multiplicity - 1.0) / 2.0;
-        for (int h = 0; h < 8; h++) {
-            dl.amopi[h] = o.amopi[h];
-            dl.poff[h] = Lq.poff[h];
-            dl.gems_ab[h] = Lq.gems_ab[h];
-            dl.gems_aa[h] = Lq.gems_aa[h];
-            dl.trip_aab[h] = Lq.trip_aab[h];
-            dl.trip_aba[h] = Lq.trip_aba[h];
-            dl.trip_aaa[h] = Lq.trip_aaa[h];
-            dl.baaa_off[h] = Lq.baaa_off[h];
-            dl.bab_off[h] = Lq.bab_off[h];
-            dl.baa_off[h] = Lq.baa_off[h];
-            dl.baab_off[h] = Lq.baab_off[h];
-            dl.baba_off[h] = Lq.baba_off[h];
-        }
         // block and row offsets of this problem, shifted to its place in the batch
-        for (int b = 0; b < v2g::XB_COUNT; b++)
-            for (int h = 0; h < 8; h++)
-                dl.xoff[b][h] = Lq.xoff[b][h] >= 0 ? Lq.xoff[b][h] + dimx : -1;
-        for (int f = 0; f < v2g::RF_COUNT; f++)
-            for (int s = 0; s < 6; s++)
-                for (int h = 0; h < 8; h++) dl.rowoff[f][s][h] = Lq.rowoff[f][s][h] + ncon;
-        dl.sym = up(Lq.sym);
-        dl.gab = up(Lq.gab);
-        dl.gaa = up(Lq.gaa);
-        dl.bab = up(Lq.bab);
-        dl.baa = up(Lq.baa);
-        dl.baab = up(Lq.baab);
-        dl.baba = up(Lq.baba);
-        dl.baaa = up(Lq.baaa);
+        const v2g::DevLayout dl = v2g::make_dev_layout(
+            Lq, {up(Lq.sym), up(Lq.gab), up(Lq.gaa), up(Lq.bab), up(Lq.baa), up(Lq.baab),
+                 up(Lq.baba), up(Lq.baaa), up(Lq.b00)},
+            dimx, ncon);
         dls.push_back(dl);
         v2g::ProbRange r{};
         r.r0 = ncon;

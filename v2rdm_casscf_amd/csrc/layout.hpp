@@ -14,6 +14,8 @@
 #include <string>
 #include <vector>
 
+struct v2g_layout_desc;
+
 namespace v2g {
 
 constexpr int kMaxIrrep = 8;
@@ -23,7 +25,10 @@ enum XBlock : int {
     XB_D2AB = 0, XB_D2AA, XB_D2BB, XB_D200, XB_D1A, XB_D1B, XB_Q1A, XB_Q1B,
     XB_Q2AB, XB_Q2AA, XB_Q2BB, XB_G2AB, XB_G2BA, XB_G2AA,
     XB_T2AAA, XB_T2BBB, XB_T2AAB, XB_T2BBA,
-    XB_T1AAA, XB_T1BBB, XB_T1AAB, XB_T1BBA, XB_COUNT
+    XB_T1AAA, XB_T1BBB, XB_T1AAB, XB_T1BBA,
+    // spin-adapted Q2 / G2 (SPIN_ADAPT_Q2 / _G2), v2rdm_solver.cc:597-635
+    XB_Q2S, XB_Q2T, XB_Q2TP1, XB_Q2TM1, XB_G2S, XB_G2T, XB_G2TP1, XB_G2TM1,
+    XB_COUNT
 };
 
 // constraint-row families in bpsdp_Au order
@@ -47,6 +52,8 @@ enum RowFam : int {
     RF_T2BIG,          // sub: 0 T2aaa(+abb), 1 T2bbb(+baa)
     RF_T1AAB,          // sub: 0 T1aab, 1 T1bba
     RF_T1AAA,          // sub: 0 T1aaa, 1 T1bbb
+    RF_Q2SA,           // spin-adapted Q2, sub: 0 Q2s, 1 Q2t, 2 Q2t(+1), 3 Q2t(-1)
+    RF_G2SA,           // spin-adapted G2, sub: 0 G2s, 1 G2t, 2 G2t(+1), 3 G2t(-1)
     RF_COUNT
 };
 
@@ -77,7 +84,15 @@ struct Options {
     int amopi[kMaxIrrep] = {0};
     int nalpha = 0, nbeta = 0, multiplicity = 1;
     bool q2 = true, g2 = true, t2 = false, spin = true, sz = true, t1 = false;
+    bool sa_q2 = false, sa_g2 = false;  // SPIN_ADAPT_Q2 / SPIN_ADAPT_G2
+    // active orbitals from lowest to highest orbital energy (order of the
+    // singlet geminals of SPIN_ADAPT_Q2); empty = Pitzer order
+    std::vector<int> eorder;
 };
+
+// the problem description of the C ABI as layout options, with the
+// reference's refusals (v2rdm_solver.cc:538-561) as std::invalid_argument
+Options options_from_desc(const v2g_layout_desc &d);
 
 class Layout {
 public:
@@ -95,6 +110,12 @@ public:
     int trip_aab[kMaxIrrep] = {0}, trip_aba[kMaxIrrep] = {0}, trip_aaa[kMaxIrrep] = {0};
     std::vector<int> baab, baba, baaa;  // packed triplets i | j << 8 | k << 16
     int64_t baab_off[kMaxIrrep] = {0}, baba_off[kMaxIrrep] = {0}, baaa_off[kMaxIrrep] = {0};
+    // singlet geminals (SPIN_ADAPT_Q2): energy-ordered pairs i | j << 16
+    int gems_00[kMaxIrrep] = {0};
+    std::vector<int> b00;
+    int64_t b00_off[kMaxIrrep] = {0};
+    bool q2_plain() const { return opt.q2 && !opt.sa_q2; }
+    bool g2_plain() const { return opt.g2 && !opt.sa_g2; }
 
     int64_t xoff[XB_COUNT][kMaxIrrep];  // -1 when the block is absent
     int64_t dimx = 0, ncon = 0;

@@ -13,14 +13,14 @@
 //   * D2ab / D2aa / D2bb entries invert each family's index rule in closed
 //     form (the particle-hole G2 maps are permutations per irrep);
 //   * D1 entries sum O(amo) rows -> one wavefront per entry.
-// With T1 or T2 on, a second launch walks the three-body rows (T1 rows in a
-// triplet pass, T2 rows in a second pass): the T1 / T2 block itself is a self
-// block, and the terms those rows put into Q2 / D2 / G2 / D1 entries are added
-// with hardware FP64 atomics, so A^T.y with T1 or T2 is not bitwise
-// reproducible from run to run (it is to rounding).
+// With T1, T2 or spin-adapted Q2 / G2 on, a second launch walks those rows:
+// each row's own block entry is a self term, and the terms the rows put into
+// Q2 / D2 / G2 / D1 / Q1 entries are added with hardware FP64 atomics, so
+// A^T.y with any of them is not bitwise reproducible from run to run (it is
+// to rounding).
 //
-// Row formulas follow d2.cc:296-547, spin.cc:274-493, q2.cc:801-950,
-// g2.cc:1061-1260 and t2.cc:1251-1861.
+// Row formulas follow d2.cc:296-547, spin.cc:274-493, q2.cc:801-950 and
+// 260-467, g2.cc:1061-1260 and 235-546, t1.cc:433-809 and t2.cc:1251-1861.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -36,9 +36,17 @@ namespace {
 
 using namespace rows;
 
+// tuning knobs (compile-time; the defaults are the measured best at H50)
+#ifndef V2G_MAP_ITEMS_PER_THREAD
+#define V2G_MAP_ITEMS_PER_THREAD 4
+#endif
+#ifndef V2G_AU_WAVES_PER_EU
+#define V2G_AU_WAVES_PER_EU 1
+#endif
+
 constexpr int kBlock = 256;
 constexpr int kWavesPerBlock = kBlock / 64;
-constexpr int kPerThread = 4;            // independent elementwise items per thread
+constexpr int kPerThread = V2G_MAP_ITEMS_PER_THREAD;   // independent elementwise items per thread
 constexpr int kElemTile = kBlock * kPerThread;
 
 __device__ __forceinline__ double wave_sum(double v)
@@ -135,7 +143,7 @@ k_g2_index(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs, i
     }
 }
 
-__global__ void __launch_bounds__(kBlock)
+__global__ void __launch_bounds__(kBlock, V2G_AU_WAVES_PER_EU)
 k_au(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
      const int64_t *__restrict__ tile_start, int nseg, const double *__restrict__ x,
      double *__restrict__ Ax, const double *__restrict__ done, const double *__restrict__ dotv,
@@ -317,7 +325,8 @@ k_atu(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
         if (ent[u] >= 0) ATy[sg.start + ent[u]] = v[u];
 }
 
-// T2 rows: self entry written, delta-selected D2 / D1 entries accumulated
+// second-pass rows (T1, T2, spin-adapted Q2 / G2): self entry written, the
+// other terms accumulated
 __global__ void __launch_bounds__(kBlock)
 k_atu_t2(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
          const int64_t *__restrict__ tile_start, int nseg, const double *__restrict__ y,
@@ -334,7 +343,7 @@ k_atu_t2(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
         const double yv = y[sg.aux + e];
         ATy[sg.start + e] = -yv;
         Scatter f{ATy, yv};
-        triplet_terms(L, sg.kind, sg.sub, sg.h, e, f, false);
+        scatter_terms(L, sg.kind, sg.sub, sg.h, e, f, false);
     }
 }
 

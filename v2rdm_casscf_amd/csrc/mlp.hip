@@ -33,8 +33,8 @@ __device__ __forceinline__ int family_group(int kind)
     case RF_SPIN_ANTI: return 4;
     case RF_SPIN_D200: case RF_SPIN_D200_4: return 5;
     case RF_MAXSPIN: return 6;
-    case RF_Q2AB: case RF_Q2AA: return 7;
-    case RF_G2AB: return 8;
+    case RF_Q2AB: case RF_Q2AA: case RF_Q2SA: return 7;
+    case RF_G2AB: case RF_G2SA: return 8;
     case RF_G2AA: return 9;
     default: return 10;
     }

@@ -3,8 +3,8 @@
 // validates every index against the CPU oracle before a kernel runs).
 //
 // Row formulas follow d2.cc:296-547, spin.cc:274-493, q2.cc:801-950,
-// g2.cc:1061-1260, t1.cc:433-809 and t2.cc:1251-1861; the A^T gathers
-// invert them.
+// g2.cc:1061-1260, t1.cc:433-809 and t2.cc:1251-1861, and the spin-adapted
+// q2.cc:260-467 and g2.cc:235-546; the A^T gathers invert the first six.
 #pragma once
 #include <cmath>
 #include <cstdint>
@@ -228,12 +228,114 @@ V2G_HD inline void t1_terms(const DevLayout &L, int kind, int s, int h, int64_t 
     }
 }
 
-// the rows walked by the triplet pass of A^T.y
+// spin-adapted Q2 (kind RF_Q2SA, q2.cc:260-467) and G2 (RF_G2SA,
+// g2.cc:235-546) rows: row e of sub-block s of irrep h.  Constants of the
+// plain rows are carried by Q1 / D1 terms, so b = 0 on all of them.
 template <class F>
-V2G_HD inline void triplet_terms(const DevLayout &L, int kind, int s, int h, int64_t e, F &f,
+V2G_HD inline void sa_terms(const DevLayout &L, int kind, int s, int h, int64_t e, F &f, bool self)
+{
+    const int amo = L.amo;
+    const int *gab = L.gab, *gaa = L.gaa, *sym = L.sym;
+    auto GAB = [&](int p, int q) -> int64_t { return gab[p * amo + q]; };
+    auto GAA = [&](int p, int q) -> int64_t { return gaa[p * amo + q]; };
+    // one-particle entry (p, q) of block w, p and q of one irrep
+    auto D1 = [&](int w, int p, int q, double c) {
+        const int h3 = sym[p], n = L.amopi[h3];
+        f(L.xoff[w][h3] + (int64_t)(p - L.poff[h3]) * n + (q - L.poff[h3]), c);
+    };
+    if (kind == RF_Q2SA) {
+        if (s == 0) {
+            // Q2s over singlet geminals (i, j), both orders with weight 1/2,
+            // q2.cc:264-319
+            const int g0 = L.gems_00[h];
+            int IJ, KL;
+            split_index(e, g0, IJ, KL);
+            const int pij = L.b00[L.b00_off[h] + IJ], pkl = L.b00[L.b00_off[h] + KL];
+            const int i = P0(pij), j = P1(pij), k = P0(pkl), l = P1(pkl);
+            if (self) f(L.xoff[XB_Q2S][h] + e, -1.0);
+            const int64_t g = L.gems_ab[h], b = L.xoff[XB_D2AB][h];
+            const int64_t ijd = GAB(i, j), jid = GAB(j, i), kld = GAB(k, l), lkd = GAB(l, k);
+            f(b + kld * g + ijd, 0.5);
+            f(b + lkd * g + ijd, 0.5);
+            f(b + kld * g + jid, 0.5);
+            f(b + lkd * g + jid, 0.5);
+            if (j == l) { D1(XB_Q1A, i, k, 0.5); D1(XB_D1B, i, k, -0.5); }
+            if (i == k) { D1(XB_Q1A, l, j, 0.5); D1(XB_D1B, l, j, -0.5); }
+            if (j == k) { D1(XB_Q1A, l, i, 0.5); D1(XB_D1B, l, i, -0.5); }
+            if (i == l) { D1(XB_Q1A, k, j, 0.5); D1(XB_D1B, k, j, -0.5); }
+            return;
+        }
+        const int ga = L.gems_aa[h];
+        int IJ, KL;
+        split_index(e, ga, IJ, KL);
+        const int pij = L.baa[L.baa_off[h] + IJ], pkl = L.baa[L.baa_off[h] + KL];
+        const int i = P0(pij), j = P1(pij), k = P0(pkl), l = P1(pkl);
+        if (s == 1) {
+            // Q2t (M_s = 0 triplet), antisymmetric D2ab combination, q2.cc:321-377
+            if (self) f(L.xoff[XB_Q2T][h] + e, -1.0);
+            const int64_t g = L.gems_ab[h], b = L.xoff[XB_D2AB][h];
+            const int64_t ijd = GAB(i, j), jid = GAB(j, i), kld = GAB(k, l), lkd = GAB(l, k);
+            f(b + kld * g + ijd, 0.5);
+            f(b + lkd * g + ijd, -0.5);
+            f(b + kld * g + jid, -0.5);
+            f(b + lkd * g + jid, 0.5);
+            if (j == l) { D1(XB_Q1A, i, k, 0.5); D1(XB_D1B, i, k, -0.5); }
+            if (i == k) { D1(XB_Q1A, l, j, 0.5); D1(XB_D1B, l, j, -0.5); }
+            if (j == k) { D1(XB_Q1A, l, i, -0.5); D1(XB_D1B, l, i, 0.5); }
+            if (i == l) { D1(XB_Q1A, k, j, -0.5); D1(XB_D1B, k, j, 0.5); }
+            return;
+        }
+        // Q2t(+1) from D2aa, Q2t(-1) from D2bb, q2.cc:378-465
+        const bool bb = s == 3;
+        const int wq = bb ? XB_Q1B : XB_Q1A, wd = bb ? XB_D1B : XB_D1A;
+        if (self) f(L.xoff[bb ? XB_Q2TM1 : XB_Q2TP1][h] + e, -1.0);
+        f(L.xoff[bb ? XB_D2BB : XB_D2AA][h] + (int64_t)KL * ga + IJ, 1.0);
+        if (j == l) D1(wq, i, k, 1.0);
+        if (j == k) D1(wd, l, i, 1.0);
+        if (i == l) D1(wq, j, k, -1.0);
+        if (i == k) D1(wd, l, j, -1.0);
+        return;
+    }
+    // spin-adapted G2 over ab pairs (i, j), (k, l)
+    const int g = L.gems_ab[h];
+    int IJ, KL;
+    split_index(e, g, IJ, KL);
+    const int pij = L.bab[L.bab_off[h] + IJ], pkl = L.bab[L.bab_off[h] + KL];
+    const int i = P0(pij), j = P1(pij), k = P0(pkl), l = P1(pkl);
+    const int h2 = sym[i] ^ sym[l];
+    const int64_t g2 = L.gems_ab[h2], b2 = L.xoff[XB_D2AB][h2];
+    if (s < 2) {
+        // G2s (s = 0) and G2t (s = 1): the like-spin blocks and the mixed
+        // ones with weight 1/2, the latter with sign +1 / -1; g2.cc:240-376
+        const double sg = s ? -0.5 : 0.5;
+        if (self) f(L.xoff[s ? XB_G2T : XB_G2S][h] + e, -1.0);
+        if (j == l) { D1(XB_D1A, i, k, 0.5); D1(XB_D1B, i, k, 0.5); }
+        if (i != l && k != j) {
+            const double c = (i < l) == (k < j) ? -0.5 : 0.5;
+            const int64_t col = GAA(i, l) * L.gems_aa[h2] + GAA(k, j);
+            f(L.xoff[XB_D2AA][h2] + col, c);
+            f(L.xoff[XB_D2BB][h2] + col, c);
+        }
+        f(b2 + GAB(i, l) * g2 + GAB(j, k), sg);
+        f(b2 + GAB(l, i) * g2 + GAB(k, j), sg);
+        return;
+    }
+    // G2t(+1) and G2t(-1): the rows of G2ab / G2ba, g2.cc:378-469
+    const bool m1 = s == 3;
+    if (self) f(L.xoff[m1 ? XB_G2TM1 : XB_G2TP1][h] + e, -1.0);
+    if (j == l) D1(m1 ? XB_D1B : XB_D1A, i, k, 1.0);
+    if (m1) f(b2 + GAB(l, i) * g2 + GAB(j, k), -1.0);
+    else f(b2 + GAB(i, l) * g2 + GAB(k, j), -1.0);
+}
+
+// the rows walked by the second pass of A^T.y (T1, T2, spin-adapted Q2 / G2):
+// the caller writes the self term, these scatter the rest
+template <class F>
+V2G_HD inline void scatter_terms(const DevLayout &L, int kind, int s, int h, int64_t e, F &f,
                                  bool self)
 {
-    if (kind == RF_T1AAB || kind == RF_T1AAA) t1_terms(L, kind, s, h, e, f, self);
+    if (kind == RF_Q2SA || kind == RF_G2SA) sa_terms(L, kind, s, h, e, f, self);
+    else if (kind == RF_T1AAB || kind == RF_T1AAA) t1_terms(L, kind, s, h, e, f, self);
     else t2_terms(L, kind, s, h, e, f, self);
 }
 
@@ -412,6 +514,10 @@ V2G_HD inline void elem_terms(const DevLayout &L, const Segment &sg, int64_t e, 
     case RF_T1AAB:
     case RF_T1AAA:
         t1_terms(L, sg.kind, s, h, e, f, true);
+        return;
+    case RF_Q2SA:
+    case RF_G2SA:
+        sa_terms(L, sg.kind, s, h, e, f, true);
         return;
     default:
         return;

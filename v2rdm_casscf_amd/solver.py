@@ -39,28 +39,52 @@ def _np_ptr(a):
     return ctypes.c_void_p(a.ctypes.data)
 
 
+def fill_desc(d, amopi, nalpha, nbeta, multiplicity=None, positivity="DQG", constrain_spin=True,
+              constrain_sz=True, spin_adapt_q2=False, spin_adapt_g2=False, energy_order=None):
+    """Fills a v2g_layout_desc; returns the energy-order array the
+    descriptor points into (keep it alive as long as the descriptor).
+
+    spin_adapt_q2 / spin_adapt_g2 are the reference's SPIN_ADAPT_Q2 /
+    SPIN_ADAPT_G2 (plugin.cc:108-110); energy_order lists the active orbitals
+    (Pitzer index) from lowest to highest orbital energy, the order of the
+    singlet geminals of spin-adapted Q2 (basis.cc:626-651), None = Pitzer."""
+    pos = positivity.upper()
+    if pos not in POSITIVITY:
+        raise ValueError(f"POSITIVITY {positivity} is not supported ({', '.join(POSITIVITY)})")
+    amopi = list(amopi)
+    if len(amopi) not in (1, 2, 4, 8):
+        raise ValueError("amopi must have 1, 2, 4 or 8 irreps")
+    d.nirrep = len(amopi)
+    for h in range(8):
+        d.amopi[h] = amopi[h] if h < len(amopi) else 0
+    d.nalpha, d.nbeta = int(nalpha), int(nbeta)
+    d.multiplicity = int(multiplicity) if multiplicity else int(nalpha - nbeta + 1)
+    d.constrain_q2, d.constrain_g2, d.constrain_t1, d.constrain_t2 = POSITIVITY[pos]
+    d.constrain_spin, d.constrain_sz = int(constrain_spin), int(constrain_sz)
+    d.spin_adapt_q2, d.spin_adapt_g2 = int(bool(spin_adapt_q2)), int(bool(spin_adapt_g2))
+    keep = None
+    if energy_order is not None:
+        eo = [int(v) for v in energy_order]
+        if sorted(eo) != list(range(sum(amopi))):
+            raise ValueError("energy_order must be a permutation of the active orbitals")
+        keep = (ctypes.c_int * len(eo))(*eo)
+        d.energy_order = ctypes.cast(keep, ctypes.POINTER(ctypes.c_int))
+    return keep
+
+
 class V2RDMSolver:
     """One active space / constraint set on one GPU (`device`)."""
 
     def __init__(self, amopi, nalpha, nbeta, multiplicity=None, positivity="DQG",
-                 constrain_spin=True, constrain_sz=True, device="cuda:0", options=None):
+                 constrain_spin=True, constrain_sz=True, device="cuda:0", options=None,
+                 spin_adapt_q2=False, spin_adapt_g2=False, energy_order=None):
         pos = positivity.upper()
-        if pos not in POSITIVITY:
-            raise ValueError(f"POSITIVITY {positivity} is not supported "
-                             f"({', '.join(POSITIVITY)})")
         amopi = list(amopi)
-        if len(amopi) not in (1, 2, 4, 8):
-            raise ValueError("amopi must have 1, 2, 4 or 8 irreps")
         self.lib = _lib.load()
         self.device = torch.device(device)
         d = _lib.LayoutDesc()
-        d.nirrep = len(amopi)
-        for h in range(8):
-            d.amopi[h] = amopi[h] if h < len(amopi) else 0
-        d.nalpha, d.nbeta = int(nalpha), int(nbeta)
-        d.multiplicity = int(multiplicity) if multiplicity else int(nalpha - nbeta + 1)
-        d.constrain_q2, d.constrain_g2, d.constrain_t1, d.constrain_t2 = POSITIVITY[pos]
-        d.constrain_spin, d.constrain_sz = int(constrain_spin), int(constrain_sz)
+        self._eorder = fill_desc(d, amopi, nalpha, nbeta, multiplicity, pos, constrain_spin,
+                                 constrain_sz, spin_adapt_q2, spin_adapt_g2, energy_order)
         self.desc = d
         self.amopi, self.nalpha, self.nbeta = amopi, nalpha, nbeta
         self.positivity = pos
