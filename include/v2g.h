@@ -72,6 +72,15 @@ int v2g_row_norm2(v2g_handle h, double *d_dev);
  * Polak-Ribiere beta.  iters receives CGSolver::total_iterations(). */
 int v2g_cg_solve(v2g_handle h, const double *rhs_dev, double *y_dev, double tol, int maxiter,
                  int precond, int flexible, int *iters, double *resnorm);
+/* The same solve without any allocation (no reference counterpart; for
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc).  The CG vectors live in a
+ * static device array of 64 MiB per GPU instead of handle buffers: problems
+ * with 3 ncon + dimx + the row-tile count (+ about 8 k) doubles beyond
+ * 8 388 608 are rejected (-1).  precond 0 or 2 only.  Calls on one GPU share
+ * that array, so calls that overlap in time on one GPU race. */
+int v2g_cg_solve_noalloc(v2g_handle h, const double *rhs_dev, double *y_dev, double tol,
+                         int maxiter, int precond, int flexible, int *iters, double *resnorm);
 /* inverse-diagonal used by precond = 2 (ncon entries, device) */
 int v2g_set_precond_diag(v2g_handle h, const double *minv_dev);
 

@@ -66,6 +66,9 @@ SIGNATURES = {
     "v2g_row_norm2": (ctypes.c_int, [_VP, _DP]),
     "v2g_cg_solve": (ctypes.c_int, [_VP, _DP, _DP, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                     ctypes.c_int, _IP, ctypes.POINTER(ctypes.c_double)]),
+    "v2g_cg_solve_noalloc": (ctypes.c_int, [_VP, _DP, _DP, ctypes.c_double, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_int, _IP,
+                                            ctypes.POINTER(ctypes.c_double)]),
     "v2g_set_precond_diag": (ctypes.c_int, [_VP, _DP]),
     "v2g_update_xz": (ctypes.c_int, [_VP, ctypes.c_double, _DP, _DP, _DP, _DP]),
     "v2g_reset_eig": (ctypes.c_int, [_VP]),

@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -38,6 +39,32 @@ namespace {
 using v2g::DevBuf;
 using v2g::HipError;
 using v2g::now;
+
+// Scratch of the allocation-free CG entry point: one array per GPU in the code
+// object's static device memory, 64 MiB.  Host code never passes `scratch`
+// itself (in host code the name is a stand-in, not a device address): every
+// handle resolves the device address once at creation, on its device, and
+// passes that pointer to the kernels.
+constexpr size_t kScratchDoubles = (size_t(64) << 20) / sizeof(double);
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_dev[kMaxDevices] = {};
+
+// device address of `scratch` on the current device (resolved once per device)
+double *scratch_for_current_device()
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) throw HipError("device index out of range for the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_dev[dev]) {
+        void *ptr = nullptr;
+        HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch_dev[dev] = static_cast<double *>(ptr);
+    }
+    return g_scratch_dev[dev];
+}
 
 constexpr int kCgChunk = 8;                 // CG iterations between host checks (even)
 constexpr int64_t kGraphMaxRows = 4 << 20;  // replay captured chunks below this many rows
@@ -114,6 +141,7 @@ struct v2g_solver {
     hipEvent_t ev_t0 = nullptr, ev_t1 = nullptr;
     v2g_iter_callback cb = nullptr;
     void *cb_user = nullptr;
+    double *scratch_dev = nullptr;          // this device's `scratch` (allocation-free CG)
     // learned preconditioner
     DevBuf<double> rownorm;
     bool have_rownorm = false;
@@ -177,6 +205,7 @@ v2g_solver::v2g_solver(const v2g_layout_desc &d)
     if (const char *e = std::getenv("V2G_CG_GRAPHS")) use_graphs = std::atoi(e) != 0;
     HIPCHK(hipStreamCreateWithFlags(&own, hipStreamNonBlocking));
     st = own;
+    scratch_dev = scratch_for_current_device();
 
     d_sym.upload(L->sym.empty() ? std::vector<int>{0} : L->sym);
     d_gab.upload(L->gab.empty() ? std::vector<int>{0} : L->gab);
@@ -844,6 +873,40 @@ int v2g_cg_solve(v2g_handle h, const double *rhs, double *y, double tol, int max
     if (!h || !rhs || !y) throw std::invalid_argument("null argument");
     if (precond < 0 || precond > 2) throw std::invalid_argument("precond must be 0, 1 or 2");
     const int it = h->cg(rhs, y, tol, maxiter, precond, flexible, resnorm);
+    if (iters) *iters = it;
+    HIPCHK(hipGetLastError());
+    API_END
+}
+
+int v2g_cg_solve_noalloc(v2g_handle h, const double *rhs, double *y, double tol, int maxiter,
+                         int precond, int flexible, int *iters, double *resnorm)
+{
+    API_BEGIN
+    if (!h || !rhs || !y) throw std::invalid_argument("null argument");
+    if (precond != 0 && precond != 2)
+        throw std::invalid_argument("the allocation-free CG takes precond 0 or 2");
+    if (precond == 2 && !h->have_user) throw std::invalid_argument("precond 2 needs v2g_set_precond_diag");
+    // scratch layout: r, p, Ap (ncon each), t (dimx), partial sums, scalars,
+    // per-tile p.Ap partials; every region starts on a 256-byte boundary
+    auto up = [](int64_t n) { return (n + 31) / 32 * 32; };
+    const int64_t ncon = h->L->ncon, dimx = h->L->dimx;
+    const int64_t need = 3 * up(ncon) + up(dimx) + up(v2g::kPartialArrays * v2g::kRedBlocks) +
+                         up(v2g::CG_NSLOTS) + up(std::max<int64_t>(1, h->plan.row_tiles));
+    if (need > (int64_t)kScratchDoubles)
+        throw std::invalid_argument("problem too large for the allocation-free CG: needs " +
+                                    std::to_string(need) + " doubles of scratch, at most " +
+                                    std::to_string(kScratchDoubles) + " (64 MiB) are available");
+    double *s = h->scratch_dev;
+    v2g_solver::CgBufs B{};
+    B.r = s;
+    B.p = B.r + up(ncon);
+    B.Ap = B.p + up(ncon);
+    B.t = B.Ap + up(ncon);
+    B.partial = B.t + up(dimx);
+    B.scal = B.partial + up(v2g::kPartialArrays * v2g::kRedBlocks);
+    B.rowpart = B.scal + up(v2g::CG_NSLOTS);
+    const double *minv = precond == 2 ? h->user_minv.p : nullptr;
+    const int it = h->cg_with(B, false, rhs, y, tol, maxiter, minv, flexible, resnorm);
     if (iters) *iters = it;
     HIPCHK(hipGetLastError());
     API_END
