@@ -241,6 +241,20 @@ int v2g_get_opdm(v2g_handle h, const double *x_dev, double *opdm_host);
 /* v2RDMSolver::get_tpdm (python_api.cc:398-458): amo^2 x amo^2, element
  * [i amo + k][j amo + l]; C1 only (-1 with the reference's message otherwise) */
 int v2g_get_tpdm(v2g_handle h, const double *x_dev, double *tpdm_host);
+/* Orbital-optimization hand-off: v2RDMSolver::PackSpatialDensity
+ * (v2rdm_solver.cc:3664-3773) from x on the device.  d1 (d1dim doubles):
+ * per irrep the packed lower triangle of D1a + D1b; d2 (d2dim doubles): per
+ * irrep of the singlet geminals (basis.cc:626-651, in the layout's orbital-
+ * energy order) the packed lower triangle over singlet-geminal pairs of the
+ * spin-summed 2-RDM, as the reference's orbital optimizer (OrbOpt,
+ * fortran.h:41-58) reads them. */
+int v2g_spatial_density_dims(v2g_handle h, long long *d1dim, long long *d2dim);
+/* the singlet geminals the packed D2 runs over: counts[h] per irrep and the
+ * pairs (i, j) (active orbitals, Pitzer order), irrep after irrep
+ * (ibas_00_sym / bas_00_sym of basis.cc:626-651) */
+int v2g_singlet_geminals(v2g_handle h, int *counts, int *pairs);
+int v2g_pack_spatial_density(v2g_handle h, const double *x_dev, double *d1_host,
+                             double *d2_host);
 /* the RDM file of v2RDMSolver::FCIDUMP (fcidump.cc:176-268): binary records
  * (double value, int i, j, k, l) of the spin-summed 2-RDM, then the 1-RDM
  * with k = l = 0; active orbitals numbered from 1 in Pitzer order (no core) */

@@ -82,6 +82,10 @@ def lib():
             getattr(L, f).restype = ctypes.c_int
             getattr(L, f).argtypes = [vp, ctypes.c_int, ctypes.c_int]
         L.orc_bas_00.argtypes = [vp, ctypes.c_int, ip]
+        for f in ("orc_spatial_d1_dim", "orc_spatial_d2_dim"):
+            getattr(L, f).restype = ctypes.c_long
+            getattr(L, f).argtypes = [vp]
+        L.orc_pack_spatial_density.argtypes = [vp, dp, dp, dp]
         L.orc_row_families.restype = ctypes.c_int
         L.orc_row_families.argtypes = [vp, ctypes.c_int, ctypes.c_char_p,
                                        ctypes.POINTER(ctypes.c_long),
@@ -193,6 +197,14 @@ class Layout:
         buf = (ctypes.c_int * max(1, 2 * n))()
         lib().orc_bas_00(self.h, h, buf)
         return [(buf[2 * a], buf[2 * a + 1]) for a in range(n)]
+
+    def pack_spatial_density(self, x):
+        """(d1, d2) packed as PackSpatialDensity (v2rdm_solver.cc:3664-3773)"""
+        d1 = np.zeros(lib().orc_spatial_d1_dim(self.h))
+        d2 = np.zeros(lib().orc_spatial_d2_dim(self.h))
+        lib().orc_pack_spatial_density(self.h, _d(np.ascontiguousarray(x, dtype=np.float64)),
+                                       _d(d1), _d(d2))
+        return d1, d2
 
     def trip_aab(self, h):
         return lib().orc_trip_aab(self.h, h)

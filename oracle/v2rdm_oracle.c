@@ -328,6 +328,107 @@ int orc_gems_ab(const orc_layout *L, int h) { return L->gems_ab[h]; }
 int orc_gems_aa(const orc_layout *L, int h) { return L->gems_aa[h]; }
 int orc_trip_aab(const orc_layout *L, int h) { return L->trip_aab[h]; }
 int orc_gems_00(const orc_layout *L, int h) { return L->gems_00[h]; }
+
+/* ------------------------------------------------------------------------ */
+/* PackSpatialDensity, v2rdm_solver.cc:3664-3773; INDEX is psi4's packed     */
+/* lower-triangle index (libciomr), max (max + 1) / 2 + min                   */
+/* ------------------------------------------------------------------------ */
+
+static long packed(long a, long b) { return a > b ? a * (a + 1) / 2 + b : b * (b + 1) / 2 + a; }
+
+long orc_spatial_d1_dim(const orc_layout *L)
+{
+    long d = 0;
+    for (int h = 0; h < L->nirrep; h++) d += (long)L->o.amopi[h] * (L->o.amopi[h] + 1) / 2;
+    return d;
+}
+
+long orc_spatial_d2_dim(const orc_layout *L)
+{
+    long d = 0;
+    for (int h = 0; h < L->nirrep; h++) d += (long)L->gems_00[h] * (L->gems_00[h] + 1) / 2;
+    return d;
+}
+
+/* index of the singlet geminal (p, q) of irrep sym p ^ sym q */
+static int sgem(const orc_layout *L, int p, int q)
+{
+    int h = L->sym[p] ^ L->sym[q];
+    for (int n = 0; n < L->gems_00[h]; n++) {
+        int a = L->b00[h][2 * n], b = L->b00[h][2 * n + 1];
+        if ((a == p && b == q) || (a == q && b == p)) return n;
+    }
+    return -1;
+}
+
+void orc_pack_spatial_density(const orc_layout *L, const double *x, double *d1, double *d2)
+{
+    const int amo = L->amo;
+    long base[9] = {0};
+    for (int h = 0; h < L->nirrep; h++)
+        base[h + 1] = base[h] + (long)L->gems_00[h] * (L->gems_00[h] + 1) / 2;
+    memset(d2, 0, sizeof(double) * base[L->nirrep]);
+    int *s00 = (int *)malloc(sizeof(int) * amo * amo);
+    for (int p = 0; p < amo; p++)
+        for (int q = 0; q < amo; q++) s00[p * amo + q] = sgem(L, p, q);
+#define AB(h, r, c) x[L->off[ORC_OFF_D2AB][h] + (long)(r) * L->gems_ab[h] + (c)]
+#define AA(w, h, r, c) x[L->off[w][h] + (long)(r) * L->gems_aa[h] + (c)]
+    for (int h = 0; h < L->nirrep; h++) {
+        for (int ij = 0; ij < L->gems_ab[h]; ij++) {
+            int i = L->bab[h][2 * ij], j = L->bab[h][2 * ij + 1];
+            int ji = L->gab[j * amo + i];
+            for (int kl = 0; kl < L->gems_ab[h]; kl++) {
+                int k = L->bab[h][2 * kl], l = L->bab[h][2 * kl + 1];
+                int lk = L->gab[l * amo + k];
+                int hik = L->sym[i] ^ L->sym[k], hkj = L->sym[k] ^ L->sym[j];
+                int ik = s00[i * amo + k], jl = s00[j * amo + l];
+                int kj = L->gab[k * amo + j], il = L->gab[i * amo + l];
+                int jk = L->gab[j * amo + k], li = L->gab[l * amo + i];
+                double dlj = (l == j) ? 0.0 : 1.0, dik = (i == k) ? 0.0 : 1.0;
+                double val = 0.0;
+                /* alpha-beta and beta-alpha, d2ab (ij,kl) and its i<->k, j<->l images */
+                val += 0.5 * AB(h, ij, kl);
+                val += 0.5 * AB(hkj, kj, il) * dlj;
+                val += 0.5 * AB(hkj, il, kj) * dik;
+                val += 0.5 * AB(h, kl, ij) * dlj * dik;
+                val += 0.5 * AB(h, ji, lk);
+                val += 0.5 * AB(hkj, jk, li) * dlj;
+                val += 0.5 * AB(hkj, li, jk) * dik;
+                val += 0.5 * AB(h, lk, ji) * dlj * dik;
+                if (i != j && k != l) {             /* alpha-alpha, beta-beta */
+                    double sg = (double)((i < j ? 1 : -1) * (k < l ? 1 : -1));
+                    int ija = L->gaa[i * amo + j], kla = L->gaa[k * amo + l];
+                    val += 0.5 * sg * AA(ORC_OFF_D2AA, h, ija, kla);
+                    val += 0.5 * sg * AA(ORC_OFF_D2AA, h, kla, ija) * dlj * dik;
+                    val += 0.5 * sg * AA(ORC_OFF_D2BB, h, ija, kla);
+                    val += 0.5 * sg * AA(ORC_OFF_D2BB, h, kla, ija) * dlj * dik;
+                }
+                if (k != j && i != l) {
+                    double sg = (double)((k < j ? 1 : -1) * (i < l ? 1 : -1));
+                    int kja = L->gaa[k * amo + j], ila = L->gaa[i * amo + l];
+                    val += 0.5 * sg * AA(ORC_OFF_D2AA, hkj, kja, ila) * dlj;
+                    val += 0.5 * sg * AA(ORC_OFF_D2AA, hkj, ila, kja) * dik;
+                    val += 0.5 * sg * AA(ORC_OFF_D2BB, hkj, kja, ila) * dlj;
+                    val += 0.5 * sg * AA(ORC_OFF_D2BB, hkj, ila, kja) * dik;
+                }
+                if (ik != jl) val *= 2.0;          /* off-diagonal elements count twice */
+                d2[base[hik] + packed(ik, jl)] = val;
+            }
+        }
+    }
+#undef AB
+#undef AA
+    free(s00);
+    long o = 0;
+    for (int h = 0; h < L->nirrep; h++) {
+        int n = L->o.amopi[h];
+        for (int i = 0; i < n; i++)
+            for (int j = i; j < n; j++)
+                d1[o + packed(i, j)] = x[L->off[ORC_OFF_D1A][h] + (long)i * n + j] +
+                                       x[L->off[ORC_OFF_D1B][h] + (long)i * n + j];
+        o += (long)n * (n + 1) / 2;
+    }
+}
 void orc_bas_00(const orc_layout *L, int h, int *pairs)
 {
     memcpy(pairs, L->b00[h], sizeof(int) * 2 * L->gems_00[h]);

@@ -25,6 +25,7 @@
  *   orc_build_c         tei.cc:144-313 (no frozen / restricted core)
  *   orc_guess_random    v2rdm_solver.cc:2479-2490 (glibc rand after srand(0))
  *   orc_bpsdp           v2rdm_solver.cc:1685-1912 (no orbital optimization)
+ *   orc_pack_spatial_density  v2rdm_solver.cc:3664-3773 (PackSpatialDensity)
  *
  * Parity status of each part is recorded in the header of v2rdm_oracle.c.
  */
@@ -94,6 +95,13 @@ enum {
     ORC_OFF_G2S, ORC_OFF_G2T, ORC_OFF_G2TP1, ORC_OFF_G2TM1,
     ORC_OFF_COUNT
 };
+
+/* PackSpatialDensity (v2rdm_solver.cc:3664-3773): spin-free active-space
+ * D1 (per irrep, packed lower triangle of D1a + D1b) and D2 (per irrep of the
+ * singlet geminals, packed lower triangle over singlet-geminal pairs) */
+long orc_spatial_d1_dim(const orc_layout *L);
+long orc_spatial_d2_dim(const orc_layout *L);
+void orc_pack_spatial_density(const orc_layout *L, const double *x, double *d1, double *d2);
 
 /* constraint maps; callee overwrites the output (bpsdp_Au / bpsdp_ATu) */
 void orc_Au(const orc_layout *L, const double *x, double *Ax);

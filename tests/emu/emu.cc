@@ -15,6 +15,7 @@
 #include "../../v2rdm_casscf_amd/csrc/devlayout.hpp"
 #include "../../v2rdm_casscf_amd/csrc/layout.hpp"
 #include "../../v2rdm_casscf_amd/csrc/rowterms.hpp"
+#include "../../v2rdm_casscf_amd/csrc/spatial.hpp"
 
 using namespace v2g;
 
@@ -168,6 +169,19 @@ void emu_ATu(void *h, const double *y, double *ATy)
     }
     for (int64_t i = 0; i < dimx; i++)
         if (!seen[i]) E->violations++;
+}
+
+// the packed spatial densities of the orbital-optimization hand-off
+void emu_spatial_dims(void *h, long long *d1, long long *d2)
+{
+    Emu *E = (Emu *)h;
+    *d1 = spatial_d1_dim(E->L);
+    *d2 = spatial_d2_dim(E->L);
+}
+
+void emu_pack_spatial_density(void *h, const double *x, double *d1, double *d2)
+{
+    pack_spatial_density(((Emu *)h)->L, x, d1, d2);
 }
 
 void emu_row_norm2(void *h, double *d)

@@ -96,6 +96,10 @@ SIGNATURES = {
     "v2g_mlp_minv": (ctypes.c_int, [_VP, _DP, _DP, _DP]),
     "v2g_get_opdm": (ctypes.c_int, [_VP, _DP, _DP]),
     "v2g_get_tpdm": (ctypes.c_int, [_VP, _DP, _DP]),
+    "v2g_spatial_density_dims": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_longlong),
+                                                ctypes.POINTER(ctypes.c_longlong)]),
+    "v2g_pack_spatial_density": (ctypes.c_int, [_VP, _DP, _DP, _DP]),
+    "v2g_singlet_geminals": (ctypes.c_int, [_VP, _IP, _IP]),
     "v2g_write_fcidump_rdm": (ctypes.c_int, [_VP, _DP, ctypes.c_char_p]),
     "v2g_batch_create": (ctypes.c_int, [ctypes.POINTER(LayoutDesc), ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_void_p)]),

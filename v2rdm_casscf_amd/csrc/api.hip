@@ -23,6 +23,7 @@
 #include "kernels.hpp"
 #include "layout.hpp"
 #include "mlp.hpp"
+#include "spatial.hpp"
 
 #include <random>
 
@@ -1365,6 +1366,50 @@ int v2g_get_tpdm(v2g_handle h, const double *x, double *tpdm)
                     }
                     tpdm[(i * a + k) * (a * a) + j * a + l] = 0.5 * (ikjl + iklj);
                 }
+    API_END
+}
+
+int v2g_spatial_density_dims(v2g_handle h, long long *d1dim, long long *d2dim)
+{
+    API_BEGIN
+    if (!h) throw std::invalid_argument("null handle");
+    if (d1dim) *d1dim = v2g::spatial_d1_dim(*h->L);
+    if (d2dim) *d2dim = v2g::spatial_d2_dim(*h->L);
+    API_END
+}
+
+int v2g_singlet_geminals(v2g_handle h, int *counts, int *pairs)
+{
+    API_BEGIN
+    if (!h) throw std::invalid_argument("null handle");
+    const v2g::Layout &L = *h->L;
+    int64_t o = 0;
+    for (int hh = 0; hh < L.nirrep; hh++) {
+        if (counts) counts[hh] = L.gems_00[hh];
+        for (int n = 0; n < L.gems_00[hh]; n++, o++) {
+            const int pr = L.b00[L.b00_off[hh] + n];
+            if (pairs) {
+                pairs[2 * o] = pr & 0xffff;
+                pairs[2 * o + 1] = pr >> 16;
+            }
+        }
+    }
+    API_END
+}
+
+int v2g_pack_spatial_density(v2g_handle h, const double *x, double *d1, double *d2)
+{
+    API_BEGIN
+    if (!h || !x || !d1 || !d2) throw std::invalid_argument("null argument");
+    const v2g::Layout &L = *h->L;
+    // the D2 and D1 blocks lead x (dimensions_ order): download up to the end of D1b
+    int64_t end = 0;
+    for (int hh = 0; hh < L.nirrep; hh++)
+        end = std::max(end, L.xoff[v2g::XB_D1B][hh] + (int64_t)L.opt.amopi[hh] * L.opt.amopi[hh]);
+    std::vector<double> xh((size_t)end);
+    HIPCHK(hipStreamSynchronize(h->st));
+    if (end) HIPCHK(hipMemcpy(xh.data(), x, sizeof(double) * end, hipMemcpyDeviceToHost));
+    v2g::pack_spatial_density(L, xh.data(), d1, d2);
     API_END
 }
 

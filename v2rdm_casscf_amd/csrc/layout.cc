@@ -102,8 +102,9 @@ void Layout::build_tables()
     }
     // singlet geminals: pairs in orbital-energy order, ieo outer and
     // jeo <= ieo inner, stored (orbital of ieo, orbital of jeo)
-    // (basis.cc:626-651)
-    if (opt.sa_q2) {
+    // (basis.cc:626-651); read by spin-adapted Q2 and by the packed spatial
+    // densities of the orbital optimizer
+    {
         std::vector<int> eo = opt.eorder;
         if (eo.empty())
             for (int i = 0; i < amo; i++) eo.push_back(i);
@@ -113,10 +114,12 @@ void Layout::build_tables()
         if (chk != id)
             throw std::invalid_argument("energy_order must be a permutation of the active orbitals");
         std::vector<std::vector<int>> per(nirrep);
+        g00.assign(amo * amo, -1);
         for (int ie = 0; ie < amo; ie++)
             for (int je = 0; je <= ie; je++) {
-                const int i = eo[ie], j = eo[je];
-                per[sym[i] ^ sym[j]].push_back(i | (j << 16));
+                const int i = eo[ie], j = eo[je], h = sym[i] ^ sym[j];
+                g00[i * amo + j] = g00[j * amo + i] = (int)per[h].size();
+                per[h].push_back(i | (j << 16));
             }
         for (int h = 0; h < nirrep; h++) {
             b00_off[h] = (int64_t)b00.size();

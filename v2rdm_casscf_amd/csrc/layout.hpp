@@ -110,9 +110,11 @@ public:
     int trip_aab[kMaxIrrep] = {0}, trip_aba[kMaxIrrep] = {0}, trip_aaa[kMaxIrrep] = {0};
     std::vector<int> baab, baba, baaa;  // packed triplets i | j << 8 | k << 16
     int64_t baab_off[kMaxIrrep] = {0}, baba_off[kMaxIrrep] = {0}, baaa_off[kMaxIrrep] = {0};
-    // singlet geminals (SPIN_ADAPT_Q2): energy-ordered pairs i | j << 16
+    // singlet geminals (SPIN_ADAPT_Q2, packed spatial densities):
+    // energy-ordered pairs i | j << 16, and the index of (i, j) in its irrep
     int gems_00[kMaxIrrep] = {0};
     std::vector<int> b00;
+    std::vector<int> g00;               // amo x amo, symmetric
     int64_t b00_off[kMaxIrrep] = {0};
     bool q2_plain() const { return opt.q2 && !opt.sa_q2; }
     bool g2_plain() const { return opt.g2 && !opt.sa_g2; }

@@ -327,6 +327,37 @@ class V2RDMSolver:
         check(self.lib.v2g_get_tpdm(self.h, _ptr(self.x), _np_ptr(out)))
         return out
 
+    # ---- orbital-optimization hand-off ------------------------------------
+    def spatial_density_dims(self):
+        d1, d2 = ctypes.c_longlong(), ctypes.c_longlong()
+        check(self.lib.v2g_spatial_density_dims(self.h, ctypes.byref(d1), ctypes.byref(d2)))
+        return d1.value, d2.value
+
+    def singlet_geminals(self):
+        """per irrep, the singlet geminals (i, j) the packed D2 runs over"""
+        nirrep = len(self.amopi)
+        counts = (ctypes.c_int * nirrep)()
+        check(self.lib.v2g_singlet_geminals(self.h, counts, None))
+        tot = sum(counts)
+        pairs = (ctypes.c_int * max(1, 2 * tot))()
+        check(self.lib.v2g_singlet_geminals(self.h, None, pairs))
+        out, o = [], 0
+        for h in range(nirrep):
+            out.append([(pairs[2 * (o + n)], pairs[2 * (o + n) + 1]) for n in range(counts[h])])
+            o += counts[h]
+        return out
+
+    def pack_spatial_density(self, x=None):
+        """v2RDMSolver::PackSpatialDensity (v2rdm_solver.cc:3664-3773): the
+        packed spin-free active D1 and D2 (numpy) of x (default: the solver's)"""
+        x = self.x if x is None else x
+        self._check_vec(x, self.dimx, "x")
+        d1dim, d2dim = self.spatial_density_dims()
+        d1, d2 = np.empty(max(1, d1dim)), np.empty(max(1, d2dim))
+        self._sync()
+        check(self.lib.v2g_pack_spatial_density(self.h, _ptr(x), _np_ptr(d1), _np_ptr(d2)))
+        return d1[:d1dim], d2[:d2dim]
+
     def write_fcidump_rdm(self, path):
         """The .rdm file of v2RDMSolver::FCIDUMP (fcidump.cc:176-268)."""
         self._sync()
