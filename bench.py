@@ -364,8 +364,10 @@ def main():
             # one line every ~20 s, so a long solve shows it is alive
             if time.perf_counter() - t_last[0] > 20.0:
                 t_last[0] = time.perf_counter()
+                gap = abs(st['energy_primal'] - st['energy_dual'])
                 print(f"# oiter {oiter} cg {st['cg_iters']} E {st['energy_primal']:.8f} "
-                      f"ep {st['ep']:.2e} ed {st['ed']:.2e}", file=sys.stderr, flush=True)
+                      f"ep {st['ep']:.2e} ed {st['ed']:.2e} gap {gap:.2e}", file=sys.stderr,
+                      flush=True)
             return False
 
         s.set_iter_callback(progress)

@@ -69,6 +69,11 @@ double *scratch_for_current_device()
 
 constexpr int kCgChunk = 8;                 // CG iterations between host checks (even)
 constexpr int64_t kGraphMaxRows = 4 << 20;  // replay captured chunks below this many rows
+// 1: a CG iteration takes p.Ap = ||A^T p||^2 from the A^T.p launch and folds
+// the residual update into the A.x launch (layouts without second-pass rows)
+#ifndef V2G_CG_FUSE
+#define V2G_CG_FUSE 1
+#endif
 
 }  // namespace
 
@@ -191,6 +196,11 @@ struct v2g_solver {
         double *r, *p, *Ap, *t, *partial, *scal, *rowpart;
     };
     CgBufs own_cg_bufs() { return {r.p, p.p, Ap.p, t.p, partial.p, scal.p, rowpart.p}; }
+    // fused iterations: rowpart holds x_tiles squares of A^T p, then three
+    // arrays of row_tiles residual partials (the unfused p.Ap partials use the
+    // first row_tiles)
+    bool fused_cg() const { return V2G_CG_FUSE && plan.t2_tiles == 0 && plan.x_tiles > 0; }
+    int64_t rowpart_len() const { return std::max<int64_t>(1, plan.x_tiles + 3 * plan.row_tiles); }
     int cg(const double *rhs, double *y, double tol, int maxiter, int precond, int flexible,
            double *res);
     int cg_with(const CgBufs &B, bool graphs, const double *rhs, double *y, double tol, int maxiter,
@@ -286,7 +296,7 @@ v2g_solver::v2g_solver(const v2g_layout_desc &d)
     Ap.alloc(ncon);
     t.alloc(dimx);
     partial.alloc(v2g::kPartialArrays * v2g::kRedBlocks);
-    rowpart.alloc(std::max<int64_t>(1, plan.row_tiles));
+    rowpart.alloc(rowpart_len());
     scal.alloc(v2g::CG_NSLOTS);
     Ax.alloc(ncon);
     Bv.alloc(ncon);
@@ -594,13 +604,31 @@ int v2g_solver::cg_with(const CgBufs &B, bool graphs, const double *rhs, double 
     // one chunk of kCgChunk (even) iterations; every kernel argument is fixed,
     // the iteration number and the done flags live in scal
     auto enqueue_chunk = [&]() {
+        const bool fused = fused_cg();
         for (int c = 0; c < kCgChunk; c++) {
             const int par = c & 1;
             const double *done = B.scal + v2g::CG_DONE0 + par;
-            v2g::launch_atu(plan, B.p, B.t, st, done);
-            v2g::launch_au(plan, B.t, B.Ap, st, done, B.p, B.rowpart);   // Ap and p.Ap
-            v2g::cg_dot_from_rows(plan.row_tiles, B.rowpart, B.partial, B.scal, par, st);
-            v2g::cg_step(n, B.r, B.Ap, minv, B.partial, B.scal, par, st);
+            if (fused) {
+                // t = A^T p with the tile squares -> alpha; r -= alpha A t in
+                // the A.x launch (A p is never stored) with the r.z, r.r
+                // partials; then y, the test and p as below
+                double *stepp = B.rowpart + plan.x_tiles;
+                v2g::launch_atu(plan, B.p, B.t, st, done, B.rowpart);
+                v2g::cg_alpha_from_tiles(plan.x_tiles, B.rowpart, B.scal, par, st);
+                v2g::CgFuse fz;
+                fz.r = B.r;
+                fz.minv = minv;
+                fz.scal = B.scal;
+                fz.part = stepp;
+                fz.ntiles = plan.row_tiles;
+                v2g::launch_au(plan, B.t, nullptr, st, done, nullptr, nullptr, fz);
+                v2g::cg_step_from_tiles(plan.row_tiles, stepp, B.partial, B.scal, par, st);
+            } else {
+                v2g::launch_atu(plan, B.p, B.t, st, done);
+                v2g::launch_au(plan, B.t, B.Ap, st, done, B.p, B.rowpart);   // Ap and p.Ap
+                v2g::cg_dot_from_rows(plan.row_tiles, B.rowpart, B.partial, B.scal, par, st);
+                v2g::cg_step(n, B.r, B.Ap, minv, B.partial, B.scal, par, st);
+            }
             v2g::cg_finish_iter(n, y, B.r, B.p, minv, B.partial, B.scal, par, flexible, st);
         }
     };
@@ -888,11 +916,12 @@ int v2g_cg_solve_noalloc(v2g_handle h, const double *rhs, double *y, double tol,
         throw std::invalid_argument("the allocation-free CG takes precond 0 or 2");
     if (precond == 2 && !h->have_user) throw std::invalid_argument("precond 2 needs v2g_set_precond_diag");
     // scratch layout: r, p, Ap (ncon each), t (dimx), partial sums, scalars,
-    // per-tile p.Ap partials; every region starts on a 256-byte boundary
+    // per-tile partials of the CG reductions; every region starts on a 256-byte
+    // boundary
     auto up = [](int64_t n) { return (n + 31) / 32 * 32; };
     const int64_t ncon = h->L->ncon, dimx = h->L->dimx;
     const int64_t need = 3 * up(ncon) + up(dimx) + up(v2g::kPartialArrays * v2g::kRedBlocks) +
-                         up(v2g::CG_NSLOTS) + up(std::max<int64_t>(1, h->plan.row_tiles));
+                         up(v2g::CG_NSLOTS) + up(h->rowpart_len());
     if (need > (int64_t)kScratchDoubles)
         throw std::invalid_argument("problem too large for the allocation-free CG: needs " +
                                     std::to_string(need) + " doubles of scratch, at most " +

@@ -203,6 +203,49 @@ k_rows_to_partials(int64_t ntiles, const double *__restrict__ rowpart, double *_
     if (threadIdx.x == 0) partial[P_PAP * kRedBlocks + blockIdx.x] = t;
 }
 
+// alpha = r.z / p.Ap with p.Ap = ||A^T p||^2 from the per-tile squares of the
+// A^T.p launch: one workgroup of kAlphaThreads sums them in a fixed order
+constexpr int kAlphaThreads = 1024;
+
+__global__ void __launch_bounds__(kAlphaThreads)
+k_alpha_tiles(int64_t ntiles, const double *__restrict__ sqpart, double *__restrict__ scal,
+              int parity)
+{
+    __shared__ double sh[kAlphaThreads / 64];
+    if (scal[S_DONE0 + parity] != 0.0) return;
+    double v = 0.0;
+    for (int64_t i = threadIdx.x; i < ntiles; i += kAlphaThreads) v += sqpart[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double pap = 0.0;
+        for (int w = 0; w < kAlphaThreads / 64; w++) pap += sh[w];
+        scal[S_ALPHA] = scal[S_RZ0 + parity] / pap;
+        scal[S_PAP] = pap;
+    }
+}
+
+// partial[P_RZ / P_RR / P_ROZ][b] = sums of contiguous chunks of the per-tile
+// r.z, r.r, r_old.z of the fused A.x launch (part holds three arrays of ntiles)
+__global__ void __launch_bounds__(kThreads)
+k_step_tiles(int64_t ntiles, const double *__restrict__ part, double *__restrict__ partial,
+             const double *__restrict__ scal, int parity)
+{
+    __shared__ double sh[kThreads / 64];
+    if (scal[S_DONE0 + parity] != 0.0) return;
+    const int64_t chunk = (ntiles + gridDim.x - 1) / gridDim.x;
+    const int64_t lo = blockIdx.x * chunk, hi = lo + chunk < ntiles ? lo + chunk : ntiles;
+    const int dst[3] = {P_RZ, P_RR, P_ROZ};
+    for (int k = 0; k < 3; k++) {
+        double s = 0.0;
+        for (int64_t i = lo + threadIdx.x; i < hi; i += kThreads) s += part[k * ntiles + i];
+        const double t = block_sum(s, sh);
+        if (threadIdx.x == 0) partial[dst[k] * kRedBlocks + blockIdx.x] = t;
+    }
+}
+
 __global__ void __launch_bounds__(kThreads)
 k_axpby(int64_t n, double a, const double *x, double b, const double *y, double *out)
 {
@@ -276,6 +319,20 @@ void cg_dot_from_rows(int64_t ntiles, const double *rowpart, double *partial, co
 {
     hipLaunchKernelGGL(k_rows_to_partials, dim3(kRedBlocks), dim3(kThreads), 0, st, ntiles, rowpart,
                        partial, scal, parity);
+}
+
+void cg_alpha_from_tiles(int64_t ntiles, const double *sqpart, double *scal, int parity,
+                         hipStream_t st)
+{
+    hipLaunchKernelGGL(k_alpha_tiles, dim3(1), dim3(kAlphaThreads), 0, st, ntiles, sqpart, scal,
+                       parity);
+}
+
+void cg_step_from_tiles(int64_t ntiles, const double *part, double *partial, const double *scal,
+                        int parity, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_step_tiles, dim3(kRedBlocks), dim3(kThreads), 0, st, ntiles, part, partial,
+                       scal, parity);
 }
 
 void cg_step(int64_t n, double *r, const double *Ap, const double *minv, double *partial,

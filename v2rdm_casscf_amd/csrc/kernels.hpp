@@ -38,12 +38,27 @@ int64_t tiles_for(const Segment &s);
 // of x are [d1lo, d1hi)
 void build_g2_index(const MapPlan &p, const std::vector<Segment> &rowsegs, int64_t d1lo,
                     int64_t d1hi, int2 *idx, hipStream_t st);
+
+// The residual update of a CG iteration fused into A.x (r is set): A.x is not
+// stored; r -= alpha (A.x) with alpha = scal[CG_ALPHA], and part[k ntiles +
+// tile] (k = 0, 1, 2) receives the tile's sums of r.z, r.r and r_old.z of the
+// new residual, z = minv r (minv null: z = r); ntiles = the plan's row_tiles
+struct CgFuse {
+    double *r = nullptr;
+    const double *minv = nullptr;
+    const double *scal = nullptr;
+    double *part = nullptr;
+    int64_t ntiles = 0;
+};
+
 // dotv / rowpart (optional): rowpart[tile] = sum over the tile's rows of dotv . Ax
 void launch_au(const MapPlan &p, const double *x, double *Ax, hipStream_t st,
                const double *done = nullptr, const double *dotv = nullptr,
-               double *rowpart = nullptr);
+               double *rowpart = nullptr, CgFuse fz = {});
+// sqpart (optional, layouts without second-pass rows only): sqpart[tile] =
+// the tile's sum of squares of A^T.y (x_tiles entries)
 void launch_atu(const MapPlan &p, const double *y, double *ATy, hipStream_t st,
-                const double *done = nullptr);
+                const double *done = nullptr, double *sqpart = nullptr);
 void launch_row_norm2(const MapPlan &p, double *d, hipStream_t st);
 
 // ---- CG (cg.hip) ----
@@ -64,6 +79,14 @@ void cg_init(int64_t n, const double *b, const double *Ap, double *r, double *p,
 // p.Ap from the per-tile partials of the fused A.x launch
 void cg_dot_from_rows(int64_t ntiles, const double *rowpart, double *partial, const double *scal,
                       int parity, hipStream_t st);
+// alpha = r.z / p.Ap of this parity with p.Ap = ||A^T p||^2 summed from the
+// per-tile squares of the A^T.p launch (one workgroup, fixed order)
+void cg_alpha_from_tiles(int64_t ntiles, const double *sqpart, double *scal, int parity,
+                         hipStream_t st);
+// the r.z, r.r and r_old.z partials of the fused A.x launch, per-tile sums
+// gathered into the partial arrays k_finish reads
+void cg_step_from_tiles(int64_t ntiles, const double *part, double *partial, const double *scal,
+                        int parity, hipStream_t st);
 // r -= alpha Ap and the partial dot products of the new residual
 void cg_step(int64_t n, double *r, const double *Ap, const double *minv, double *partial,
              double *scal, int parity, hipStream_t st);

@@ -24,6 +24,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <stdexcept>
 #include <vector>
 
 #include "devlayout.hpp"
@@ -113,6 +114,20 @@ __device__ __forceinline__ int find_segment(const int64_t *__restrict__ tile_sta
     return lo;
 }
 
+// block sum of one value per thread in a fixed order (wave butterflies, then
+// the waves in order); the result is valid in thread 0
+__device__ __forceinline__ double block_total(double v, double *red)
+{
+    v = wave_sum(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < kWavesPerBlock; w++) s += red[w];
+    __syncthreads();
+    return s;
+}
+
 // Column indices of the G2 rows (65 % of all rows at large active spaces),
 // built once per layout: for the row visited at position pos of a G2
 // segment, g2idx[aux + pos] = (D2 column << 1 | coefficient sign, or -1 when
@@ -143,11 +158,13 @@ k_g2_index(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs, i
     }
 }
 
+// fz.r (optional): the CG residual update fused in, r -= alpha (A.x) with the
+// tile sums of the new residual's dot products instead of A.x itself
 __global__ void __launch_bounds__(kBlock, V2G_AU_WAVES_PER_EU)
 k_au(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
      const int64_t *__restrict__ tile_start, int nseg, const double *__restrict__ x,
      double *__restrict__ Ax, const double *__restrict__ done, const double *__restrict__ dotv,
-     double *__restrict__ rowpart, const int2 *__restrict__ g2idx, int done_stride)
+     double *__restrict__ rowpart, const int2 *__restrict__ g2idx, int done_stride, CgFuse fz)
 {
     __shared__ double red[kWavesPerBlock];
     const int64_t tile = xcd_tile(blockIdx.x, gridDim.x);
@@ -157,7 +174,23 @@ k_au(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
     if (done && done[(int64_t)sg.prob * done_stride] != 0.0) return;
     const DevLayout &L = Lp[sg.prob];
     const int64_t t = tile - tile_start[si];
-    double contrib = 0.0;
+    const double alpha = fz.r ? fz.scal[CG_ALPHA] : 0.0;
+    double contrib = 0.0, rz = 0.0, rr = 0.0, roz = 0.0;
+    // row i of A.x has the value v
+    auto emit = [&](int64_t i, double v) {
+        if (fz.r) {
+            const double ro = fz.r[i];
+            const double rn = ro - alpha * v;
+            fz.r[i] = rn;
+            const double zn = fz.minv ? fz.minv[i] * rn : rn;
+            rz += rn * zn;
+            rr += rn * rn;
+            roz += ro * zn;
+        } else {
+            Ax[i] = v;
+            if (dotv) contrib += v * dotv[i];
+        }
+    };
     if (sg.wave) {
         const int64_t item = t * kWavesPerBlock + (threadIdx.x >> 6);
         if (item < sg.count) {
@@ -165,10 +198,7 @@ k_au(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
             Gather f{x, 0.0};
             wave_terms(L, sg, item, lane, f);
             const double v = wave_sum(f.acc);
-            if (lane == 0) {
-                Ax[sg.start + item] = v;
-                if (dotv) contrib = v * dotv[sg.start + item];
-            }
+            if (lane == 0) emit(sg.start + item, v);
         }
     } else if (g2idx && sg.aux >= 0) {
         // G2 rows: self entry, D2 term and D1 term from the precomputed columns
@@ -200,12 +230,8 @@ k_au(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
             }
         }
 #pragma unroll
-        for (int u = 0; u < kPerThread; u++) {
-            if (row[u] >= 0) {
-                Ax[sg.start + row[u]] = v[u];
-                if (dotv) contrib += v[u] * dotv[sg.start + row[u]];
-            }
-        }
+        for (int u = 0; u < kPerThread; u++)
+            if (row[u] >= 0) emit(sg.start + row[u], v[u]);
     } else {
         // kPerThread independent rows per thread: their gathers overlap in flight
         double v[kPerThread];
@@ -219,23 +245,23 @@ k_au(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
             v[u] = f.acc;
         }
 #pragma unroll
-        for (int u = 0; u < kPerThread; u++) {
-            if (row[u] >= 0) {
-                Ax[sg.start + row[u]] = v[u];
-                if (dotv) contrib += v[u] * dotv[sg.start + row[u]];
-            }
-        }
+        for (int u = 0; u < kPerThread; u++)
+            if (row[u] >= 0) emit(sg.start + row[u], v[u]);
     }
-    if (rowpart) {
-        // this workgroup's share of dotv . Ax, in a fixed order
-        contrib = wave_sum(contrib);
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = contrib;
-        __syncthreads();
+    if (fz.r) {
+        // this workgroup's shares of r.z, r.r and r_old.z, in a fixed order
+        const double a = block_total(rz, red);
+        const double b2 = block_total(rr, red);
+        const double c = block_total(roz, red);
         if (threadIdx.x == 0) {
-            double s2 = 0.0;
-            for (int w = 0; w < kWavesPerBlock; w++) s2 += red[w];
-            rowpart[tile] = s2;
+            fz.part[tile] = a;
+            fz.part[fz.ntiles + tile] = b2;
+            fz.part[2 * fz.ntiles + tile] = c;
         }
+    } else if (rowpart) {
+        // this workgroup's share of dotv . Ax, in a fixed order
+        const double s2 = block_total(contrib, red);
+        if (threadIdx.x == 0) rowpart[tile] = s2;
     }
 }
 
@@ -281,12 +307,16 @@ k_row_norm2(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
     }
 }
 
-// eight waves per SIMD (57 VGPRs) to hide the gathers' latency
+// eight waves per SIMD (57 VGPRs) to hide the gathers' latency.  sqpart
+// (optional): sqpart[tile] = sum of the squares of the tile's entries of
+// A^T.y, so that p.(A A^T p) = ||A^T p||^2 needs no second read of p
 __global__ void __launch_bounds__(kBlock, 8)
 k_atu(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
       const int64_t *__restrict__ tile_start, int nseg, const double *__restrict__ y,
-      double *__restrict__ ATy, const double *__restrict__ done, int64_t ntiles, int done_stride)
+      double *__restrict__ ATy, const double *__restrict__ done, int64_t ntiles, int done_stride,
+      double *__restrict__ sqpart)
 {
+    __shared__ double red[kWavesPerBlock];
     // chunked (not contiguous) XCD ranges: the primal segments differ widely in
     // cost (closed-form D2 gathers vs copies), and contiguous per-XCD ranges
     // would leave some XCDs with all of the expensive tiles
@@ -297,32 +327,44 @@ k_atu(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
     if (done && done[(int64_t)sg.prob * done_stride] != 0.0) return;
     const DevLayout &L = Lp[sg.prob];
     const int64_t t = tile - tile_start[si];
+    double sq = 0.0;
     if (sg.wave) {
         const int64_t item = t * kWavesPerBlock + (threadIdx.x >> 6);
-        if (item >= sg.count) return;
-        const int lane = threadIdx.x & 63;
-        const double v = wave_sum(atu_d1_wave(L, sg, item, lane, y));
-        if (lane == 0) ATy[sg.start + item] = v;
-        return;
-    }
-    double v[kPerThread];
-    int64_t ent[kPerThread];
-#pragma unroll
-    for (int u = 0; u < kPerThread; u++) {
-        const int64_t pos = t * kElemTile + u * kBlock + threadIdx.x;
-        v[u] = 0.0;
-        ent[u] = pos < sg.count ? entry_of(L, sg, pos) : -1;
-        if (ent[u] >= 0) {
-            switch (sg.kind) {
-            case XS_D2AB: v[u] = atu_d2ab(L, sg, ent[u], y); break;
-            case XS_D2AA: v[u] = atu_d2aa(L, sg, ent[u], y); break;
-            default: v[u] = atu_self(L, sg, ent[u], y); break;
+        if (item < sg.count) {
+            const int lane = threadIdx.x & 63;
+            const double v = wave_sum(atu_d1_wave(L, sg, item, lane, y));
+            if (lane == 0) {
+                ATy[sg.start + item] = v;
+                sq = v * v;
             }
         }
-    }
+    } else {
+        double v[kPerThread];
+        int64_t ent[kPerThread];
 #pragma unroll
-    for (int u = 0; u < kPerThread; u++)
-        if (ent[u] >= 0) ATy[sg.start + ent[u]] = v[u];
+        for (int u = 0; u < kPerThread; u++) {
+            const int64_t pos = t * kElemTile + u * kBlock + threadIdx.x;
+            v[u] = 0.0;
+            ent[u] = pos < sg.count ? entry_of(L, sg, pos) : -1;
+            if (ent[u] >= 0) {
+                switch (sg.kind) {
+                case XS_D2AB: v[u] = atu_d2ab(L, sg, ent[u], y); break;
+                case XS_D2AA: v[u] = atu_d2aa(L, sg, ent[u], y); break;
+                default: v[u] = atu_self(L, sg, ent[u], y); break;
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kPerThread; u++)
+            if (ent[u] >= 0) {
+                ATy[sg.start + ent[u]] = v[u];
+                sq += v[u] * v[u];
+            }
+    }
+    if (sqpart) {
+        const double s = block_total(sq, red);
+        if (threadIdx.x == 0) sqpart[tile] = s;
+    }
 }
 
 // second-pass rows (T1, T2, spin-adapted Q2 / G2): self entry written, the
@@ -350,12 +392,12 @@ k_atu_t2(const DevLayout *__restrict__ Lp, const Segment *__restrict__ segs,
 }  // namespace
 
 void launch_au(const MapPlan &p, const double *x, double *Ax, hipStream_t st, const double *done,
-               const double *dotv, double *rowpart)
+               const double *dotv, double *rowpart, CgFuse fz)
 {
     if (p.row_tiles > 0)
         hipLaunchKernelGGL(k_au, dim3((unsigned)p.row_tiles), dim3(kBlock), 0, st, p.dlay,
                            p.rowsegs, p.row_tile_start, p.nrowsegs, x, Ax, done, dotv, rowpart,
-                           p.g2idx, p.done_stride);
+                           p.g2idx, p.done_stride, fz);
 }
 
 void build_g2_index(const MapPlan &p, const std::vector<Segment> &rowsegs, int64_t d1lo,
@@ -377,11 +419,17 @@ void launch_row_norm2(const MapPlan &p, double *d, hipStream_t st)
                            p.rowsegs, p.row_tile_start, p.nrowsegs, d);
 }
 
-void launch_atu(const MapPlan &p, const double *y, double *ATy, hipStream_t st, const double *done)
+void launch_atu(const MapPlan &p, const double *y, double *ATy, hipStream_t st, const double *done,
+                double *sqpart)
 {
+    // the squares are summed as the entries are written: only valid when no
+    // second pass adds to them afterwards
+    if (sqpart && p.t2_tiles > 0)
+        throw std::logic_error("||A^T y||^2 partials need a layout without second-pass rows");
     if (p.x_tiles > 0)
         hipLaunchKernelGGL(k_atu, dim3((unsigned)chunk_grid(p.x_tiles)), dim3(kBlock), 0, st, p.dlay,
-                           p.xsegs, p.x_tile_start, p.nxsegs, y, ATy, done, p.x_tiles, p.done_stride);
+                           p.xsegs, p.x_tile_start, p.nxsegs, y, ATy, done, p.x_tiles, p.done_stride,
+                           sqpart);
     if (p.t2_tiles > 0)
         hipLaunchKernelGGL(k_atu_t2, dim3((unsigned)p.t2_tiles), dim3(kBlock), 0, st, p.dlay,
                            p.t2segs, p.t2_tile_start, p.nt2segs, y, ATy, done, p.done_stride);
