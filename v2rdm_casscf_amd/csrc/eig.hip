@@ -570,48 +570,12 @@ k_pair_eig(const EigBlock *__restrict__ blocks, int round, const double *__restr
     }
 }
 
-// 64x64x64 product on FP64 MFMA: out = op(X) Y with row pitches ldx, ldy
-// (an operand may sit in LDS or in global memory); transX: X is read
-// transposed.  Wave w owns the 32x32 output sub-tile (rows 32 (w/2), cols
-// 32 (w%2)); v_mfma_f64_16x16x4f64 operand layout: A[i][k] from lane i + 16k,
-// B[k][j] from lane j + 16k, D[i][j] in lane j + 16 (i%4), element i/4 (one
-// FP64 row per accumulator register).
-__device__ __forceinline__ void mfma_tile(const double *X, int ldx, const double *Y, int ldy,
-                                          bool transX, v4d (&acc)[2][2])
-{
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-    const int li = lane & 15, lk = lane >> 4;
-    for (int a = 0; a < 2; a++)
-        for (int b = 0; b < 2; b++) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
-#pragma unroll 4
-    for (int k0 = 0; k0 < kP; k0 += 4) {
-        const int k = k0 + lk;
-        double av[2], bv[2];
-        for (int a = 0; a < 2; a++) {
-            const int row = wr + a * 16 + li;
-            av[a] = transX ? X[k * ldx + row] : X[row * ldx + k];
-        }
-        for (int b = 0; b < 2; b++) bv[b] = Y[k * ldy + wc + b * 16 + li];
-        for (int a = 0; a < 2; a++)
-            for (int b = 0; b < 2; b++)
-                acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
-    }
-}
-
-__device__ __forceinline__ void store_tile(double *T, const v4d (&acc)[2][2])
-{
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-    for (int a = 0; a < 2; a++)
-        for (int b = 0; b < 2; b++)
-            for (int r = 0; r < 4; r++)
-                T[(wr + a * 16 + 4 * r + (lane >> 4)) * kLd + wc + b * 16 + (lane & 15)] = acc[a][b][r];
-}
-
 // A_PR <- Q_P^T A_PR Q_R for the pair tiles with P <= R (A is symmetric and
-// kept as its upper triangle of 32 x 32 sub-blocks); V[:, R] <- V[:, R] Q_R.  Four waves per SIMD: the LDS tile allows four
-// workgroups per CU, and the bound keeps the registers within that (116 VGPRs)
+// kept as its upper triangle of 32 x 32 sub-blocks); V[:, R] <- V[:, R] Q_R.
+// Four waves per SIMD: the LDS tile allows four workgroups per CU, and the
+// bound keeps the registers within that (122 VGPRs).  FP64 MFMA
+// v_mfma_f64_16x16x4f64: A[i][k] from lane i + 16k, B[k][j] from lane
+// j + 16k, D[i][j] in lane j + 16 (i % 4), element i / 4.
 __global__ void __launch_bounds__(kT, 4)
 k_apply(const EigBlock *__restrict__ blocks, int round, double *__restrict__ Aw,
         double *__restrict__ Vw, const double *__restrict__ Qbuf, const int *__restrict__ active,
@@ -695,42 +659,100 @@ k_apply(const EigBlock *__restrict__ blocks, int round, double *__restrict__ Aw,
 #pragma unroll
         for (int u = 0; u < kP * kP / kT; u++) T[(i0 + u * (kT / kP)) * kLd + j] = v[u];
     }
+    // The tile passes through LDS once.  Wave w forms the 64 x 16 column strip
+    // w of U = T Q_R (A operand T from LDS, B operand Q_R from global memory)
+    // in its MFMA accumulators: accumulator register r of row block bi holds
+    // U[16 bi + 4 r + lane / 16][16 w + lane % 16], which is exactly the B
+    // operand layout of the k-step 16 bi + 4 r of the next product, so the
+    // strip of Q_P^T U is formed from registers.  Stored sub-blocks are then
+    // written from registers (four 128-byte row pieces per store); stale
+    // (transposed) ones go through LDS for coalesced writes.
     __syncthreads();
-    v4d acc[2][2];
-    if (actP) {
-        const double *QP = Qbuf + B.qoff + (int64_t)P * kP * kP;
-        mfma_tile(QP, kP, T, kLd, true, acc);      // Q_P^T T
-        __syncthreads();
-        store_tile(T, acc);
-        __syncthreads();
-    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int li = lane & 15, lk = lane >> 4;
+    const int jc = 16 * wave + li;
+    v4d u[4];
     if (actR) {
         const double *QR = Qbuf + B.qoff + (int64_t)R * kP * kP;
-        mfma_tile(T, kLd, QR, kP, false, acc);     // T Q_R
-        __syncthreads();
-        store_tile(T, acc);
-    }
-    __syncthreads();
-    if (isA) {
 #pragma unroll
-        for (int s = 0; s < 4; s++) {
-            if (P == R && s == 2) continue;   // (pb, pa) of a diagonal tile
+        for (int bi = 0; bi < 4; bi++) u[bi] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+        for (int k0 = 0; k0 < kP; k0 += 4) {
+            const double bv = QR[(k0 + lk) * kP + jc];
 #pragma unroll
-            for (int k = 0; k < 4; k++) {
-                int ti, tj;
-                const int64_t a = sub_addr(s >> 1, s & 1, k, ti, tj);
-                M[a] = T[ti * kLd + tj];
+            for (int bi = 0; bi < 4; bi++) {
+                const double av = T[(16 * bi + li) * kLd + k0 + lk];
+                u[bi] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, u[bi], 0, 0, 0);
             }
         }
     } else {
 #pragma unroll
-        for (int u = 0; u < kP * kP / kT; u++) {
-            const int i = i0 + u * (kT / kP);
-            M[(int64_t)(rt * kP + i) * B.N + colj] = T[i * kLd + j];
+        for (int bi = 0; bi < 4; bi++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) u[bi][r] = T[(16 * bi + 4 * r + lk) * kLd + jc];
+    }
+    v4d o[4];
+    if (actP) {
+        const double *QP = Qbuf + B.qoff + (int64_t)P * kP * kP;
+#pragma unroll
+        for (int bi = 0; bi < 4; bi++) o[bi] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int kb = 0; kb < 4; kb++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int k = 16 * kb + 4 * r + lk;
+                const double bv = u[kb][r];
+#pragma unroll
+                for (int bi = 0; bi < 4; bi++) {
+                    const double av = QP[k * kP + 16 * bi + li];   // (Q_P^T)[row][k]
+                    o[bi] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, o[bi], 0, 0, 0);
+                }
+            }
+    } else {
+#pragma unroll
+        for (int bi = 0; bi < 4; bi++) o[bi] = u[bi];
+    }
+    if (!isA) {
+        const int64_t col = pair_index(ra, rb, jc);
+#pragma unroll
+        for (int bi = 0; bi < 4; bi++)
+#pragma unroll
+            for (int r = 0; r < 4; r++)
+                M[(int64_t)(rt * kP + 16 * bi + 4 * r + lk) * B.N + col] = o[bi][r];
+        return;
+    }
+    __syncthreads();   // every wave has read T
+    const int vb = wave >> 1, vv = vb ? rb : ra;
+#pragma unroll
+    for (int ub = 0; ub < 2; ub++) {
+        const int uu = ub ? pb : pa;
+        if (P == R && ub == 1 && vb == 0) continue;   // (pb, pa) of a diagonal tile
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int bi = 2 * ub + h;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int i = 16 * bi + 4 * r + lk;
+                if (uu <= vv)
+                    M[(int64_t)(uu * 32 + i - 32 * ub) * B.N + vv * 32 + jc - 32 * vb] = o[bi][r];
+                else
+                    T[i * kLd + jc] = o[bi][r];
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const int ub = s >> 1, vb2 = s & 1;
+        const int uu = ub ? pb : pa, v2 = vb2 ? rb : ra;
+        if (uu <= v2 || (P == R && s == 2)) continue;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const int sr = sr0 + 8 * k;
+            M[(int64_t)(v2 * 32 + sr) * B.N + uu * 32 + sc] = T[(ub * 32 + sc) * kLd + vb2 * 32 + sr];
         }
     }
 }
-
 // T[:, j] = V[:, j] sqrt(max(lambda_j, 0) / mu), lambda_j = A[j][j]
 __global__ void __launch_bounds__(kT)
 k_scale_vecs(const EigBlock *__restrict__ blocks, const double *__restrict__ Aw,
