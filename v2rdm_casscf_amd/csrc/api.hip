@@ -69,6 +69,11 @@ double *scratch_for_current_device()
 
 constexpr int kCgChunk = 8;                 // CG iterations between host checks (even)
 constexpr int64_t kGraphMaxRows = 4 << 20;  // replay captured chunks below this many rows
+// cyclic sweeps of each 64 x 64 pair problem per round in a split that must
+// converge (a direct Update_xz, a cold split, the exact re-split)
+#ifndef V2G_EIG_FULL_INNER
+#define V2G_EIG_FULL_INNER 12
+#endif
 // 1: a CG iteration takes p.Ap = ||A^T p||^2 from the A^T.p launch and folds
 // the residual update into the A.x launch (layouts without second-pass rows)
 #ifndef V2G_CG_FUSE
@@ -716,7 +721,7 @@ void v2g_solver::update_xz(double mu, const double *y, const double *c, double *
         v2g::gemm_batched(d_gemm_warm2.p, nl, max_N, max_N, st);
     }
     const int maxsweeps = (!cold && warm_sweep_cap > 0) ? warm_sweep_cap : 40;
-    const int inner = (!cold && inner_sweep_cap > 0) ? inner_sweep_cap : 12;
+    const int inner = (!cold && inner_sweep_cap > 0) ? inner_sweep_cap : V2G_EIG_FULL_INNER;
     int sweeps = 0, swept = 0;
     if (timed) HIPCHK(hipEventRecord(ev_t0, st));
     for (; sweeps < maxsweeps; sweeps++) {
@@ -770,7 +775,7 @@ void v2g_solver::update_xz(double mu, const double *y, const double *c, double *
         }
     }
     last_sweeps = sweeps;
-    last_split_capped = !cold && sweeps > 0 && (sweeps >= maxsweeps || inner < 12);
+    last_split_capped = !cold && sweeps > 0 && (sweeps >= maxsweeps || inner < V2G_EIG_FULL_INNER);
     // x = (V s)(V s)^T with s = sqrt(max(lambda,0)/mu); z = mu x - M
     v2g::eig_scale(d_large.p, nl, Aw.p, Vw.p, Tw.p, mu, st);
     // the reconstruction descriptors point into x: built once per x buffer
@@ -1054,9 +1059,15 @@ int v2g_bpsdp(v2g_handle h, const v2g_params *prm, const double *c, const double
     double last_loss = 0.0;
     do {
         const double t1 = now();
-        // Ax = -tau mu (A x - b); z = c - z; B = A z + Ax  (v2rdm_solver.cc:1784-1794)
-        v2g::launch_au(h->plan, x, h->Ax.p, st);
-        v2g::vec_axpby(ncon, -tau * mu, h->Ax.p, tau * mu, b, h->Ax.p, st);
+        // Ax = -tau mu (A x - b); z = c - z; B = A z + Ax  (v2rdm_solver.cc:1784-1794).
+        // After the first iteration of a call, Ax already holds A x - b for this x
+        // (the error evaluation of the previous iteration)
+        if (oiter == oiter_first) {
+            v2g::launch_au(h->plan, x, h->Ax.p, st);
+            v2g::vec_axpby(ncon, -tau * mu, h->Ax.p, tau * mu, b, h->Ax.p, st);
+        } else {
+            v2g::vec_axpby(ncon, -tau * mu, h->Ax.p, 0.0, h->Ax.p, h->Ax.p, st);
+        }
         v2g::vec_axpby(dimx, 1.0, c, -1.0, z, z, st);
         v2g::launch_au(h->plan, z, h->Bv.p, st);
         v2g::vec_axpby(ncon, 1.0, h->Bv.p, 1.0, h->Ax.p, h->Bv.p, st);
@@ -1139,9 +1150,9 @@ int v2g_bpsdp(v2g_handle h, const v2g_params *prm, const double *c, const double
                      prm->eig_pair_order);
         HIPCHK(hipStreamSynchronize(st));
         txz += now() - t2;
-        // ed = ||A^T y - c + z||, ep = ||A x - b||  (v2rdm_solver.cc:1829-1837)
+        // ed = ||A^T y - c + z||, ep = ||A x - b||  (v2rdm_solver.cc:1829-1837).
+        // Every call follows an update_xz with this y, which left A^T y in ATy
         auto errors = [&]() {
-            v2g::launch_atu(h->plan, y, h->ATy.p, st);
             v2g::vec_axpby(dimx, 1.0, h->ATy.p, 1.0, z, h->ATy.p, st);
             v2g::vec_axpby(dimx, 1.0, h->ATy.p, -1.0, c, h->ATy.p, st);
             ed = std::sqrt(h->dot(h->ATy.p, h->ATy.p, dimx));
