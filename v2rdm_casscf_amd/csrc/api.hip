@@ -67,13 +67,12 @@ double *scratch_for_current_device()
     return g_scratch_dev[dev];
 }
 
+#ifndef V2G_EIG_FULL_CROSS
+#define V2G_EIG_FULL_CROSS 0
+#endif
+
 constexpr int kCgChunk = 8;                 // CG iterations between host checks (even)
 constexpr int64_t kGraphMaxRows = 4 << 20;  // replay captured chunks below this many rows
-// cyclic sweeps of each 64 x 64 pair problem per round in a split that must
-// converge (a direct Update_xz, a cold split, the exact re-split)
-#ifndef V2G_EIG_FULL_INNER
-#define V2G_EIG_FULL_INNER 12
-#endif
 // 1: a CG iteration takes p.Ap = ||A^T p||^2 from the A^T.p launch and folds
 // the residual update into the A.x launch (layouts without second-pass rows)
 #ifndef V2G_CG_FUSE
@@ -721,7 +720,7 @@ void v2g_solver::update_xz(double mu, const double *y, const double *c, double *
         v2g::gemm_batched(d_gemm_warm2.p, nl, max_N, max_N, st);
     }
     const int maxsweeps = (!cold && warm_sweep_cap > 0) ? warm_sweep_cap : 40;
-    const int inner = (!cold && inner_sweep_cap > 0) ? inner_sweep_cap : V2G_EIG_FULL_INNER;
+    const int inner = (!cold && inner_sweep_cap > 0) ? inner_sweep_cap : v2g::kEigFullInner;
     int sweeps = 0, swept = 0;
     if (timed) HIPCHK(hipEventRecord(ev_t0, st));
     for (; sweeps < maxsweeps; sweeps++) {
@@ -738,7 +737,7 @@ void v2g_solver::update_xz(double mu, const double *y, const double *c, double *
                 hipStream_t s = l ? lane_st[l] : st;
                 // a capped tracking sweep may take the cross-pair order after
                 // its first round (every pair is still visited once)
-                const int bip = (capped && cross_order && rnd > 0) ? 1 : 0;
+                const int bip = ((capped ? cross_order : V2G_EIG_FULL_CROSS) && rnd > 0) ? 1 : 0;
                 v2g::eig_pairs(bl, lane_n[l], lane_pairs[l], rnd, Aw.p, Qbuf.p, active.p,
                                tol.p + lane_off[l], counter.p, inner, pair_wide ? 1 : 0, bip, s);
                 if (rnd == 0 && l % 2 == 0 && l + 1 < nlanes) {
@@ -775,7 +774,7 @@ void v2g_solver::update_xz(double mu, const double *y, const double *c, double *
         }
     }
     last_sweeps = sweeps;
-    last_split_capped = !cold && sweeps > 0 && (sweeps >= maxsweeps || inner < V2G_EIG_FULL_INNER);
+    last_split_capped = !cold && sweeps > 0 && (sweeps >= maxsweeps || inner < v2g::kEigFullInner);
     // x = (V s)(V s)^T with s = sqrt(max(lambda,0)/mu); z = mu x - M
     v2g::eig_scale(d_large.p, nl, Aw.p, Vw.p, Tw.p, mu, st);
     // the reconstruction descriptors point into x: built once per x buffer

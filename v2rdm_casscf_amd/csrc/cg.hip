@@ -213,8 +213,17 @@ k_alpha_tiles(int64_t ntiles, const double *__restrict__ sqpart, double *__restr
 {
     __shared__ double sh[kAlphaThreads / 64];
     if (scal[S_DONE0 + parity] != 0.0) return;
-    double v = 0.0;
-    for (int64_t i = threadIdx.x; i < ntiles; i += kAlphaThreads) v += sqpart[i];
+    // four independent sums per thread keep four loads in flight
+    double v0 = 0.0, v1 = 0.0, v2 = 0.0, v3 = 0.0;
+    int64_t i = threadIdx.x;
+    for (; i + 3 * kAlphaThreads < ntiles; i += 4 * kAlphaThreads) {
+        v0 += sqpart[i];
+        v1 += sqpart[i + kAlphaThreads];
+        v2 += sqpart[i + 2 * kAlphaThreads];
+        v3 += sqpart[i + 3 * kAlphaThreads];
+    }
+    for (; i < ntiles; i += kAlphaThreads) v0 += sqpart[i];
+    double v = (v0 + v1) + (v2 + v3);
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
