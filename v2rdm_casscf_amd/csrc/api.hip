@@ -67,10 +67,6 @@ double *scratch_for_current_device()
     return g_scratch_dev[dev];
 }
 
-#ifndef V2G_EIG_FULL_CROSS
-#define V2G_EIG_FULL_CROSS 0
-#endif
-
 constexpr int kCgChunk = 8;                 // CG iterations between host checks (even)
 constexpr int64_t kGraphMaxRows = 4 << 20;  // replay captured chunks below this many rows
 // 1: a CG iteration takes p.Ap = ||A^T p||^2 from the A^T.p launch and folds
@@ -735,9 +731,10 @@ void v2g_solver::update_xz(double mu, const double *y, const double *c, double *
                 if (rnd >= lane_nb[l] - 1) continue;
                 const v2g::EigBlock *bl = d_large.p + lane_off[l];
                 hipStream_t s = l ? lane_st[l] : st;
-                // a capped tracking sweep may take the cross-pair order after
-                // its first round (every pair is still visited once)
-                const int bip = ((capped ? cross_order : V2G_EIG_FULL_CROSS) && rnd > 0) ? 1 : 0;
+                // after its first round a sweep takes the cross-pair order
+                // (every pair is still visited once per sweep): always in a
+                // split that must converge, by option in a capped tracking sweep
+                const int bip = ((capped ? cross_order : 1) && rnd > 0) ? 1 : 0;
                 v2g::eig_pairs(bl, lane_n[l], lane_pairs[l], rnd, Aw.p, Qbuf.p, active.p,
                                tol.p + lane_off[l], counter.p, inner, pair_wide ? 1 : 0, bip, s);
                 if (rnd == 0 && l % 2 == 0 && l + 1 < nlanes) {

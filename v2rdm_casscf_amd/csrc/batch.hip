@@ -39,10 +39,6 @@
 #include "kernels.hpp"
 #include "layout.hpp"
 
-#ifndef V2G_EIG_FULL_CROSS
-#define V2G_EIG_FULL_CROSS 0
-#endif
-
 namespace v2g {
 namespace {
 
@@ -693,7 +689,7 @@ void v2g_batch::update_xz(const std::vector<double> &mu, const std::vector<int> 
     for (int sweep = 0; sweep < maxsweeps; sweep++) {
         HIPCHK(hipMemsetAsync(counter.p, 0, sizeof(int), st));
         for (int rnd = 0; rnd < max_nb - 1; rnd++) {
-            const int bip = ((capped ? cross_order : V2G_EIG_FULL_CROSS) && rnd > 0) ? 1 : 0;
+            const int bip = ((capped ? cross_order : 1) && rnd > 0) ? 1 : 0;
             v2g::eig_pairs(d_large.p, nl, max_pairs, rnd, Aw.p, Qbuf.p, active.p, tol.p, counter.p,
                            inner, pair_wide ? 1 : 0, bip, st, eb);
             v2g::eig_apply(d_large.p, nl, max_tiles, rnd, Aw.p, Vw.p, Qbuf.p, active.p, st, eb);

@@ -10,8 +10,9 @@ namespace v2g {
 // cyclic sweeps of each 64 x 64 pair problem per round in a split that must
 // converge (a direct Update_xz, a cold split, the exact re-split): the block
 // sweeps repeat until none rotates, so one pass per pair problem and round
-// reaches the same tolerance in the least time (H50 warm split: 175 ms with 1,
-// 219 ms with 2, 349 ms with 12)
+// reaches the same tolerance in the least time (H50 warm split with the
+// cyclic order in every round: 175 ms with 1, 219 ms with 2, 349 ms with 12;
+// 143 ms with 1 and the cross-pair order after each sweep's first round)
 constexpr int kEigFullInner = 1;
 
 struct EigBlock {
