@@ -301,7 +301,7 @@ v2g_solver::v2g_solver(const v2g_layout_desc &d)
     Ax.alloc(ncon);
     Bv.alloc(ncon);
     ATy.alloc(dimx);
-    red.alloc(v2g::kRedBlocks + 8);
+    red.alloc(4 * v2g::kRedBlocks + 8);   // up to four partial arrays, then their sums
 
     // eigen-split plan: blocks <= 64 whole in one workgroup, others block Jacobi
     int64_t woff = 0, qoff = 0, poff = 0;
@@ -1148,15 +1148,23 @@ int v2g_bpsdp(v2g_handle h, const v2g_params *prm, const double *c, const double
         txz += now() - t2;
         // ed = ||A^T y - c + z||, ep = ||A x - b||  (v2rdm_solver.cc:1829-1837).
         // Every call follows an update_xz with this y, which left A^T y in ATy
+        // The four sums are formed on the device and read back together.
         auto errors = [&]() {
-            v2g::vec_axpby(dimx, 1.0, h->ATy.p, 1.0, z, h->ATy.p, st);
-            v2g::vec_axpby(dimx, 1.0, h->ATy.p, -1.0, c, h->ATy.p, st);
-            ed = std::sqrt(h->dot(h->ATy.p, h->ATy.p, dimx));
+            constexpr int64_t R = v2g::kRedBlocks;
+            double *red = h->red.p;
+            v2g::vec_lincomb_sq(dimx, 1.0, h->ATy.p, 1.0, z, -1.0, c, h->ATy.p, red, st);
             v2g::launch_au(h->plan, x, h->Ax.p, st);
-            v2g::vec_axpby(ncon, 1.0, h->Ax.p, -1.0, b, h->Ax.p, st);
-            ep = std::sqrt(h->dot(h->Ax.p, h->Ax.p, ncon));
-            ecur = h->dot(c, x, dimx);
-            edual = h->dot(b, y, ncon);
+            v2g::vec_lincomb_sq(ncon, 1.0, h->Ax.p, -1.0, b, 0.0, nullptr, h->Ax.p, red + R, st);
+            v2g::vec_dot_partial(dimx, c, x, red + 2 * R, st);
+            v2g::vec_dot_partial(ncon, b, y, red + 3 * R, st);
+            for (int q = 0; q < 4; q++) v2g::vec_reduce(red + q * R, red + 4 * R + q, st);
+            double o[4];
+            HIPCHK(hipMemcpyAsync(o, red + 4 * R, sizeof(o), hipMemcpyDeviceToHost, st));
+            HIPCHK(hipStreamSynchronize(st));
+            ed = std::sqrt(o[0]);
+            ep = std::sqrt(o[1]);
+            ecur = o[2];
+            edual = o[3];
         };
         errors();
         auto passes = [&]() {

@@ -263,6 +263,25 @@ k_axpby(int64_t n, double a, const double *x, double b, const double *y, double 
         out[i] = a * x[i] + (y ? b * y[i] : 0.0);
 }
 
+// out = a x + b y + cw w (w null: no third term) and the block partial sums
+// of out.out (the grid and summation order of k_dotp)
+__global__ void __launch_bounds__(kThreads)
+k_lincomb_sq(int64_t n, double a, const double *x, double b, const double *y, double cw,
+             const double *w, double *out, double *__restrict__ partial)
+{
+    __shared__ double sh[kThreads / 64];
+    double s = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)kThreads + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * kThreads) {
+        double v = a * x[i] + b * y[i];
+        if (w) v = v + cw * w[i];
+        out[i] = v;
+        s += v * v;
+    }
+    const double t = block_sum(s, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
 __global__ void __launch_bounds__(kThreads)
 k_fillv(int64_t n, double v, double *out)
 {
@@ -362,6 +381,13 @@ void vec_axpby(int64_t n, double a, const double *x, double b, const double *y, 
                hipStream_t st)
 {
     hipLaunchKernelGGL(k_axpby, dim3(kRedBlocks), dim3(kThreads), 0, st, n, a, x, b, y, out);
+}
+
+void vec_lincomb_sq(int64_t n, double a, const double *x, double b, const double *y, double cw,
+                    const double *w, double *out, double *partial, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_lincomb_sq, dim3(kRedBlocks), dim3(kThreads), 0, st, n, a, x, b, y, cw, w,
+                       out, partial);
 }
 
 void vec_fill(int64_t n, double v, double *out, hipStream_t st)

@@ -100,6 +100,10 @@ void cg_finish_iter(int64_t n, double *y, double *r, double *p, const double *z,
 void vec_axpby(int64_t n, double a, const double *x, double b, const double *y, double *out,
                hipStream_t st);
 void vec_fill(int64_t n, double v, double *out, hipStream_t st);
+// out = a x + b y + cw w (w may be null; out may alias x) and kRedBlocks partial sums of
+// out.out for vec_reduce
+void vec_lincomb_sq(int64_t n, double a, const double *x, double b, const double *y, double cw,
+                    const double *w, double *out, double *partial, hipStream_t st);
 void vec_dot_partial(int64_t n, const double *a, const double *b, double *partial, hipStream_t st);
 void vec_reduce(const double *partial, double *out, hipStream_t st);
 // y <- 2 y - yh and yh <- old y (linear extrapolation of the CG starting guess)
