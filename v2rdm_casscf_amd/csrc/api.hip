@@ -1049,7 +1049,7 @@ int v2g_bpsdp(v2g_handle h, const v2g_params *prm, const double *c, const double
     std::mt19937_64 rng(20240607ULL);
     long long cg_train = 0;
     int n_train = 0;
-    // the A/B guard state lives in the handle, so a later call keeps the verdict
+    // the A/B guard state lives in the handle, so a later call keeps its decision
     bool &fallback = h->mlp_fallback, &first_learned = h->mlp_first_learned;
     int &last_base_iters = h->mlp_last_base_iters;
     double last_loss = 0.0;
